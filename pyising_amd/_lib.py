@@ -1,0 +1,202 @@
+"""ctypes binding of ``libising_hip.so``, the C ABI over the gfx950 kernels.
+
+The library is built in-tree by :mod:`pyising_amd.build` (``__graft_entry__.build()``). Every
+entry point returns 0 or a negative status; :func:`check` turns a failure into an exception
+carrying ``ising_last_error()``. There is no CPU fallback: if the library is missing or no GPU
+is visible, :func:`lib` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libising_hip.so"
+
+_P = C.c_void_p
+_I = C.c_int
+_LL = C.c_longlong
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_SZ = C.c_size_t
+
+#: argument types of every exported entry point (mirrors include/ising_hip.h)
+SIGNATURES: dict[str, list] = {
+    "ising_device_count": [_P],
+    "ising_device_info": [_P, _I, _P, _P],
+    "ising_malloc": [_P, _SZ],
+    "ising_free": [_P],
+    "ising_mem_info": [_P, _P],
+    "ising_memcpy_h2d": [_P, _P, _SZ, _P],
+    "ising_memcpy_d2h": [_P, _P, _SZ, _P],
+    "ising_memcpy_d2d": [_P, _P, _SZ, _P],
+    "ising_stream_create": [_P],
+    "ising_stream_destroy": [_P],
+    "ising_stream_sync": [_P],
+    "ising_device_sync": [],
+    "ising_event_create": [_P],
+    "ising_event_destroy": [_P],
+    "ising_event_record": [_P, _P],
+    "ising_event_query": [_P],
+    "ising_event_elapsed_ms": [_P, _P, _P],
+    "ising_philox_kat": [_P, _P, _P, _I],
+    "ising_philox_rate": [_P, _I, _I, _U32, _U32, _P],
+    "ising_copy": [_P, _P, _SZ, _I, _P],
+    "ising_fill_u64": [_P, _LL, _U64, _P],
+    "ising_split_int8": [_P, _P, _P, _LL, _I, _I, _P],
+    "ising_merge_int8": [_P, _P, _P, _LL, _I, _I, _P],
+    "ising_split_pack": [_P, _P, _P, _LL, _I, _P],
+    "ising_unpack_merge": [_P, _P, _P, _LL, _I, _P],
+    "ising_int8_half": [_P, _P, _LL, _I, _I, _I, _I, _I, _P, _U32, _U32, _U32, _U32, _P],
+    "ising_int8_sweeps": [_P, _P, _LL, _I, _I, _I, _I, _I, _P, _U32, _LL, _U32, _U32, _P],
+    "ising_int8_obs": [_P, _P, _LL, _I, _I, _I, _I, _P, _P, _P],
+    "ising_msc_config": [_I, _I],
+    "ising_msc_half": [_P, _P, _LL, _I, _U32, _U32, _U32, _U32, _U32, _U32, _P, _P],
+    "ising_msc_sweeps": [_P, _P, _LL, _I, _U32, _LL, _U32, _U32, _U32, _U32, _P, _P],
+    "ising_msc_half_ex": [_P, _P, _LL, _I, _U32, _U32, _U32, _U32, _U32, _U32, _I, _I, _P, _P],
+    "ising_msc_sweeps_ex": [_P, _P, _LL, _I, _U32, _LL, _U32, _U32, _U32, _U32, _I, _I, _P, _P],
+    "ising_msc_obs": [_P, _P, _LL, _I, _P, _P, _P],
+}
+
+
+class HipError(RuntimeError):
+    """A C-ABI entry point returned a failure status."""
+
+
+_lib: C.CDLL | None = None
+_lock = threading.Lock()
+
+
+def load_library() -> C.CDLL:
+    """Load the shared library and declare its signatures (no GPU needed)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                                  "python -m pyising_amd.build")
+            lib = C.CDLL(str(LIB_PATH))
+            for name, args in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = args
+                fn.restype = _I
+            lib.ising_last_error.argtypes = []
+            lib.ising_last_error.restype = C.c_char_p
+            _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc < 0:
+        msg = load_library().ising_last_error().decode(errors="replace")
+        raise HipError(f"{what or 'ising call'} failed ({rc}): {msg}")
+    return rc
+
+
+def device_count() -> int:
+    """Number of visible GPUs (0 when the runtime reports none or fails)."""
+    n = C.c_int(0)
+    try:
+        rc = load_library().ising_device_count(C.byref(n))
+    except (ImportError, OSError):
+        return 0
+    return n.value if rc == 0 else 0
+
+
+def lib() -> C.CDLL:
+    """The library, with at least one GPU visible; raises otherwise (no CPU fallback)."""
+    L = load_library()
+    if device_count() < 1:
+        raise HipError("no HIP device visible: the hip backend runs only on a GPU")
+    return L
+
+
+class DeviceBuffer:
+    """Owned device allocation (freed on garbage collection)."""
+
+    __slots__ = ("ptr", "nbytes", "_lib")
+
+    def __init__(self, nbytes: int):
+        self._lib = lib()
+        p = C.c_void_p()
+        check(self._lib.ising_malloc(C.byref(p), int(nbytes)), "ising_malloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            self._lib.ising_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:  # noqa: BLE001 - interpreter teardown
+            pass
+
+    def upload(self, host, stream=None) -> None:
+        import numpy as np
+
+        a = np.ascontiguousarray(host)
+        if a.nbytes > self.nbytes:
+            raise ValueError("upload larger than the buffer")
+        check(self._lib.ising_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes, stream), "h2d")
+
+    def download(self, host, stream=None) -> None:
+        if host.nbytes > self.nbytes or not host.flags["C_CONTIGUOUS"]:
+            raise ValueError("download target must be contiguous and fit the buffer")
+        check(self._lib.ising_memcpy_d2h(host.ctypes.data, self.ptr, host.nbytes, stream), "d2h")
+
+
+class Event:
+    """HIP event (timing on a given stream)."""
+
+    def __init__(self):
+        self._lib = lib()
+        p = C.c_void_p()
+        check(self._lib.ising_event_create(C.byref(p)), "event_create")
+        self.ptr = p.value
+
+    def record(self, stream=None) -> None:
+        check(self._lib.ising_event_record(self.ptr, stream), "event_record")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(self._lib.ising_event_elapsed_ms(C.byref(ms), self.ptr, end.ptr), "event_elapsed")
+        return float(ms.value)
+
+    def done(self) -> bool:
+        return check(self._lib.ising_event_query(self.ptr), "event_query") == 1
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self._lib.ising_event_destroy(self.ptr)
+        except Exception:  # noqa: BLE001
+            pass
+
+
+class Stream:
+    """Non-blocking HIP stream at the default priority."""
+
+    def __init__(self):
+        self._lib = lib()
+        p = C.c_void_p()
+        check(self._lib.ising_stream_create(C.byref(p)), "stream_create")
+        self.ptr = p.value
+
+    def sync(self) -> None:
+        check(self._lib.ising_stream_sync(self.ptr), "stream_sync")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self._lib.ising_stream_destroy(self.ptr)
+        except Exception:  # noqa: BLE001
+            pass
+
+
+def device_sync() -> None:
+    check(lib().ising_device_sync(), "device_sync")

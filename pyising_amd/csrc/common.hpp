@@ -1,0 +1,82 @@
+// Shared definitions of the gfx950 Ising library: status handling for the C ABI, the
+// in-register Philox4x32-10, and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+// ---------------------------------------------------------------- C-ABI status handling ----
+// Every entry point returns 0 on success or a negative code; the message of the last failure
+// on the calling thread is available from ising_last_error().
+enum {
+    ISING_OK = 0,
+    ISING_EINVAL = -1,   // bad argument (shape, size, pointer)
+    ISING_EHIP = -2,     // a HIP runtime call failed
+    ISING_ERANGE = -3,   // a size beyond what the entry point supports
+};
+
+void ising_set_error(const std::string& msg);
+int ising_fail(int code, const std::string& msg);
+int ising_check_hip(hipError_t e, const char* what);
+
+#define ISING_HIP(call)                                              \
+    do {                                                             \
+        hipError_t _e = (call);                                      \
+        if (_e != hipSuccess) return ising_check_hip(_e, #call);     \
+    } while (0)
+
+#define ISING_LAUNCHED(what)                                         \
+    do {                                                             \
+        hipError_t _e = hipGetLastError();                           \
+        if (_e != hipSuccess) return ising_check_hip(_e, what);      \
+    } while (0)
+
+// ------------------------------------------------------------------------------ Philox ----
+// Philox4x32-10 (Salmon et al., SC'11). Round: (hi0, lo0) = M0 * c0, (hi1, lo1) = M1 * c2,
+// c <- (hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0), then the key is bumped by (W0, W1). The
+// 32x32->64 products compile to one v_mad_u64_u32 each; where a counter word is uniform across
+// the wave (sweep, colour) the compiler keeps the early rounds' products on the scalar unit.
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+struct PhiloxOut {
+    u32 x, y, z, w;
+};
+
+__device__ __forceinline__ PhiloxOut philox4x32_10(u32 c0, u32 c1, u32 c2, u32 c3, u32 k0, u32 k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const u64 p0 = (u64)c0 * PHILOX_M0;
+        const u64 p1 = (u64)c2 * PHILOX_M1;
+        const u32 n0 = (u32)(p1 >> 32) ^ c1 ^ k0;
+        const u32 n2 = (u32)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = (u32)p1;
+        c2 = n2;
+        c3 = (u32)p0;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+    return PhiloxOut{c0, c1, c2, c3};
+}
+
+// --------------------------------------------------------------------- launch helpers ----
+// MI355X dispatches workgroup i to XCD i % 8. For stencil kernels whose neighbouring blocks
+// share halo rows, remap so that each XCD walks one contiguous band of blocks: rows then
+// stay in that XCD's L2. Valid when nblocks % 8 == 0 (otherwise identity).
+__device__ __forceinline__ u32 xcd_band_block(u32 b, u32 nblocks)
+{
+    if (nblocks & 7u) return b;
+    const u32 per = nblocks >> 3;
+    return (b & 7u) * per + (b >> 3);
+}
+
+static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }

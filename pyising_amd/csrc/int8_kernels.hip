@@ -1,0 +1,412 @@
+// int8 compressed-checkerboard half sweeps (2D/3D, Metropolis and heat bath) and the exact
+// integer observables of that layout, for gfx950.
+//
+// Layout (restates src/pyising/lattice.py:1-14 of the reference): colour c of the lattice is
+// a row-major (rows, lh) int8 array, lh = Lx / 2, rows = Ly (2D) or Lz * Ly (3D); in row
+// r = z * Ly + y the colour-c site j sits at x = 2 j + ((y + z + c) & 1). Spins are the bytes
+// 0x01 (+1) and 0xFF (-1).
+//
+// Work split: one thread owns V consecutive colour sites of one row (V = 16, 8 or 4, the
+// largest that divides lh) and loads them as whole 16/8/4-byte vectors: own, the other colour
+// at the same j, the other colour at y +- 1 (and z +- 1 in 3D), plus one edge byte for the
+// horizontal neighbour. Neighbour counting is SWAR over 4 sites per 32-bit word: bit 1 of a
+// spin byte is 1 exactly for -1, so ((a ^ b) >> 1) & 0x01010101 marks anti-aligned pairs and
+// (b >> 1) & 0x01010101 marks -1 neighbours, 4 sites per instruction, no carries between
+// bytes (counts <= 6).
+//
+// Random numbers (src/pyising/philox.py:54-64): colour-site n takes word n % 4 of
+// Philox4x32-10((u32)(n / 4), t, colour, 0; k0, k1), so each 4-site word of a thread's vector
+// costs exactly one Philox call.
+//
+// Decision rules (src/pyising/backends/numpy_backend.py:35-44, src/pyising/lattice.py:128-142):
+//   Metropolis, a = anti-aligned neighbours: flip iff a >= d or u < table[2d - a]
+//   heat bath, m = -1 neighbours:            new spin +1 iff u < table[2d - m]
+// The table is passed by value (scalar registers); per-site selection is a short v_cndmask
+// chain, never an indexed memory load.
+#include "common.hpp"
+
+struct Int8Table {
+    u32 v[7];  // table[0 .. 2d]; unused entries zero
+};
+
+struct Int8Geom {
+    u32 lh;          // colour sites per row
+    u32 ly, lz;      // rows = lz * ly (lz = 1 in 2D)
+    u32 chunks_row;  // lh / V
+    u64 n_chunks;    // rows * lh / V
+};
+
+template <int V>
+struct Vec;
+template <>
+struct Vec<4> {
+    u32 w[1];
+};
+template <>
+struct Vec<8> {
+    u32 w[2];
+};
+template <>
+struct Vec<16> {
+    u32 w[4];
+};
+
+template <int V>
+__device__ __forceinline__ Vec<V> load_vec(const int8_t* p)
+{
+    Vec<V> r;
+    if constexpr (V == 16) {
+        const uint4 q = *reinterpret_cast<const uint4*>(p);
+        r.w[0] = q.x; r.w[1] = q.y; r.w[2] = q.z; r.w[3] = q.w;
+    } else if constexpr (V == 8) {
+        const uint2 q = *reinterpret_cast<const uint2*>(p);
+        r.w[0] = q.x; r.w[1] = q.y;
+    } else {
+        r.w[0] = *reinterpret_cast<const u32*>(p);
+    }
+    return r;
+}
+
+template <int V>
+__device__ __forceinline__ void store_vec(int8_t* p, const Vec<V>& r)
+{
+    if constexpr (V == 16) {
+        *reinterpret_cast<uint4*>(p) = make_uint4(r.w[0], r.w[1], r.w[2], r.w[3]);
+    } else if constexpr (V == 8) {
+        *reinterpret_cast<uint2*>(p) = make_uint2(r.w[0], r.w[1]);
+    } else {
+        *reinterpret_cast<u32*>(p) = r.w[0];
+    }
+}
+
+#define LOW_BITS 0x01010101u
+
+template <int NDIM>
+__device__ __forceinline__ u32 metro_thr(u32 a, const Int8Table& tb)
+{
+    // only a < d can be accepted by the comparison; a >= d always flips
+    if constexpr (NDIM == 2) {
+        return a == 0 ? tb.v[4] : tb.v[3];
+    } else {
+        return a == 0 ? tb.v[6] : (a == 1 ? tb.v[5] : tb.v[4]);
+    }
+}
+
+template <int NDIM>
+__device__ __forceinline__ u32 heat_thr(u32 m, const Int8Table& tb)
+{
+    constexpr int Z = 2 * NDIM;
+    u32 thr = tb.v[Z];
+#pragma unroll
+    for (int k = 1; k <= Z; ++k) thr = (m == (u32)k) ? tb.v[Z - k] : thr;
+    return thr;
+}
+
+// One thread: sites j0 .. j0 + V - 1 of row `row`.
+template <int NDIM, int HB, int V>
+__global__ __launch_bounds__(256) void int8_half_kernel(int8_t* __restrict__ own,
+                                                        const int8_t* __restrict__ other,
+                                                        Int8Geom g, Int8Table tb, u32 t, u32 color,
+                                                        u32 k0, u32 k1)
+{
+    constexpr int NWd = V / 4;
+    const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
+    const u64 ci = (u64)blk * blockDim.x + threadIdx.x;
+    if (ci >= g.n_chunks) return;
+    const u32 row = (u32)(ci / g.chunks_row);
+    const u32 j0 = (u32)(ci - (u64)row * g.chunks_row) * V;
+    const u32 z = row / g.ly;
+    const u32 y = row - z * g.ly;
+    const u32 p = (y + z + color) & 1u;
+    const u64 rbase = (u64)row * g.lh;
+    const u64 n0 = rbase + j0;
+
+    const Vec<V> S = load_vec<V>(own + n0);
+    const Vec<V> C = load_vec<V>(other + n0);
+    const u32 yu = (y + 1 == g.ly) ? 0u : y + 1;
+    const u32 yd = (y == 0) ? g.ly - 1 : y - 1;
+    const Vec<V> U = load_vec<V>(other + ((u64)z * g.ly + yu) * g.lh + j0);
+    const Vec<V> D = load_vec<V>(other + ((u64)z * g.ly + yd) * g.lh + j0);
+    Vec<V> ZU, ZD;
+    if constexpr (NDIM == 3) {
+        const u32 zu = (z + 1 == g.lz) ? 0u : z + 1;
+        const u32 zd = (z == 0) ? g.lz - 1 : z - 1;
+        ZU = load_vec<V>(other + ((u64)zu * g.ly + y) * g.lh + j0);
+        ZD = load_vec<V>(other + ((u64)zd * g.ly + y) * g.lh + j0);
+    }
+    // horizontal neighbour: j + 1 when p == 1, j - 1 when p == 0 (one edge byte from memory)
+    const u32 je = p ? ((j0 + V == g.lh) ? 0u : j0 + V) : ((j0 == 0) ? g.lh - 1 : j0 - 1);
+    const u32 e = (u32)(uint8_t)other[rbase + je];
+    Vec<V> H;
+    if (p) {
+#pragma unroll
+        for (int i = 0; i < NWd; ++i) {
+            const u32 nxt = (i + 1 < NWd) ? C.w[i + 1 < NWd ? i + 1 : 0] : e;
+            H.w[i] = (C.w[i] >> 8) | (nxt << 24);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NWd; ++i) {
+            const u32 prv = (i > 0) ? (C.w[i > 0 ? i - 1 : 0] >> 24) : e;
+            H.w[i] = (C.w[i] << 8) | prv;
+        }
+    }
+
+    Vec<V> O;
+#pragma unroll
+    for (int i = 0; i < NWd; ++i) {
+        const u32 q = (u32)((n0 >> 2) + (u64)i);  // quad index, truncated to 32 bits
+        const PhiloxOut r = philox4x32_10(q, t, color, 0u, k0, k1);
+        const u32 uu[4] = {r.x, r.y, r.z, r.w};
+        u32 cnt;
+        if constexpr (HB) {
+            cnt = ((C.w[i] >> 1) & LOW_BITS) + ((H.w[i] >> 1) & LOW_BITS) +
+                  ((U.w[i] >> 1) & LOW_BITS) + ((D.w[i] >> 1) & LOW_BITS);
+            if constexpr (NDIM == 3)
+                cnt += ((ZU.w[i] >> 1) & LOW_BITS) + ((ZD.w[i] >> 1) & LOW_BITS);
+        } else {
+            const u32 s = S.w[i];
+            cnt = (((s ^ C.w[i]) >> 1) & LOW_BITS) + (((s ^ H.w[i]) >> 1) & LOW_BITS) +
+                  (((s ^ U.w[i]) >> 1) & LOW_BITS) + (((s ^ D.w[i]) >> 1) & LOW_BITS);
+            if constexpr (NDIM == 3)
+                cnt += (((s ^ ZU.w[i]) >> 1) & LOW_BITS) + (((s ^ ZD.w[i]) >> 1) & LOW_BITS);
+        }
+        u32 out = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const u32 c = (cnt >> (8 * b)) & 0xFFu;
+            if constexpr (HB) {
+                const bool up = uu[b] < heat_thr<NDIM>(c, tb);
+                out |= (up ? 0x01u : 0xFFu) << (8 * b);
+            } else {
+                const bool flip = (c >= (u32)NDIM) || (uu[b] < metro_thr<NDIM>(c, tb));
+                out |= (flip ? 0xFEu : 0x00u) << (8 * b);
+            }
+        }
+        O.w[i] = HB ? out : (S.w[i] ^ out);
+    }
+    store_vec<V>(own + n0, O);
+}
+
+// ------------------------------------------------------------------------ observables ----
+// M = sum of all spins = N - 2 * (#-1); E = -sum over colour-0 sites of s * h =
+// 2 * (#anti-aligned bonds) - d * N (every bond has exactly one colour-0 end,
+// src/pyising/backends/observables_torch.py:1-4). Accumulated as exact integers: per-lane
+// counts, wave reduction with cross-lane adds, one 64-bit atomic per wave.
+__device__ __forceinline__ u64 wave_sum_u64(u64 v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ u32 byte_sum(u32 v)  // sum of the 4 bytes (each <= 63)
+{
+    return (v * LOW_BITS) >> 24;
+}
+
+template <int NDIM, int V>
+__global__ __launch_bounds__(256) void int8_obs_kernel(const int8_t* __restrict__ c0,
+                                                       const int8_t* __restrict__ c1, Int8Geom g,
+                                                       unsigned long long* __restrict__ acc)
+{
+    constexpr int NWd = V / 4;
+    u64 minus = 0, anti = 0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 ci = (u64)blockIdx.x * blockDim.x + threadIdx.x; ci < g.n_chunks; ci += stride) {
+        const u32 row = (u32)(ci / g.chunks_row);
+        const u32 j0 = (u32)(ci - (u64)row * g.chunks_row) * V;
+        const u32 z = row / g.ly;
+        const u32 y = row - z * g.ly;
+        const u32 p = (y + z) & 1u;  // colour 0
+        const u64 rbase = (u64)row * g.lh;
+        const u64 n0 = rbase + j0;
+        const Vec<V> S = load_vec<V>(c0 + n0);
+        const Vec<V> C = load_vec<V>(c1 + n0);
+        const u32 yu = (y + 1 == g.ly) ? 0u : y + 1;
+        const u32 yd = (y == 0) ? g.ly - 1 : y - 1;
+        const Vec<V> U = load_vec<V>(c1 + ((u64)z * g.ly + yu) * g.lh + j0);
+        const Vec<V> D = load_vec<V>(c1 + ((u64)z * g.ly + yd) * g.lh + j0);
+        Vec<V> ZU, ZD;
+        if constexpr (NDIM == 3) {
+            const u32 zu = (z + 1 == g.lz) ? 0u : z + 1;
+            const u32 zd = (z == 0) ? g.lz - 1 : z - 1;
+            ZU = load_vec<V>(c1 + ((u64)zu * g.ly + y) * g.lh + j0);
+            ZD = load_vec<V>(c1 + ((u64)zd * g.ly + y) * g.lh + j0);
+        }
+        const u32 je = p ? ((j0 + V == g.lh) ? 0u : j0 + V) : ((j0 == 0) ? g.lh - 1 : j0 - 1);
+        const u32 e = (u32)(uint8_t)c1[rbase + je];
+        u32 cm = 0, ca = 0;
+#pragma unroll
+        for (int i = 0; i < NWd; ++i) {
+            u32 h;
+            if (p) {
+                const u32 nxt = (i + 1 < NWd) ? C.w[i + 1 < NWd ? i + 1 : 0] : e;
+                h = (C.w[i] >> 8) | (nxt << 24);
+            } else {
+                const u32 prv = (i > 0) ? (C.w[i > 0 ? i - 1 : 0] >> 24) : e;
+                h = (C.w[i] << 8) | prv;
+            }
+            const u32 s = S.w[i];
+            cm += byte_sum(((s >> 1) & LOW_BITS) + ((C.w[i] >> 1) & LOW_BITS));
+            u32 a = (((s ^ C.w[i]) >> 1) & LOW_BITS) + (((s ^ h) >> 1) & LOW_BITS) +
+                    (((s ^ U.w[i]) >> 1) & LOW_BITS) + (((s ^ D.w[i]) >> 1) & LOW_BITS);
+            if constexpr (NDIM == 3)
+                a += (((s ^ ZU.w[i]) >> 1) & LOW_BITS) + (((s ^ ZD.w[i]) >> 1) & LOW_BITS);
+            ca += byte_sum(a);
+        }
+        minus += cm;
+        anti += ca;
+    }
+    minus = wave_sum_u64(minus);
+    anti = wave_sum_u64(anti);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(acc + 0, (unsigned long long)minus);
+        atomicAdd(acc + 1, (unsigned long long)anti);
+    }
+}
+
+// out[0] = M = N - 2 minus; out[1] = E = 2 anti - d N. Also clears the accumulator so the
+// next call starts from zero without a memset.
+__global__ void obs_finalize_kernel(unsigned long long* acc, long long* out, long long n_spins,
+                                    int ndim)
+{
+    const long long minus = (long long)acc[0], anti = (long long)acc[1];
+    out[0] = n_spins - 2 * minus;
+    out[1] = 2 * anti - (long long)ndim * n_spins;
+    acc[0] = 0;
+    acc[1] = 0;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------- host launchers ----
+static int pick_vec(u32 lh)
+{
+    if (lh % 16 == 0) return 16;
+    if (lh % 8 == 0) return 8;
+    if (lh % 4 == 0) return 4;
+    return 0;
+}
+
+static int make_geom(long long rows, int lh, int ly, int lz, int ndim, Int8Geom* g, int* V)
+{
+    if (rows <= 0 || lh <= 0 || ly <= 0 || lz <= 0 || (ndim != 2 && ndim != 3))
+        return ising_fail(ISING_EINVAL, "int8: bad geometry");
+    if ((long long)ly * lz != rows) return ising_fail(ISING_EINVAL, "int8: rows != ly * lz");
+    if (ndim == 2 && lz != 1) return ising_fail(ISING_EINVAL, "int8: 2D requires lz == 1");
+    const int v = pick_vec((u32)lh);
+    if (!v) return ising_fail(ISING_EINVAL, "int8: lh must be a multiple of 4 (Lx % 8 == 0)");
+    if (ly < 2 || lh < 4 || (ndim == 3 && lz < 2)) return ising_fail(ISING_EINVAL, "int8: extents too small");
+    g->lh = (u32)lh;
+    g->ly = (u32)ly;
+    g->lz = (u32)lz;
+    g->chunks_row = (u32)(lh / v);
+    g->n_chunks = (u64)rows * (u64)(lh / v);
+    if (g->n_chunks / 256 + 1 > 0x7FFFFFFFull)
+        return ising_fail(ISING_ERANGE, "int8: lattice too large for one launch");
+    *V = v;
+    return ISING_OK;
+}
+
+template <int NDIM, int HB>
+static void launch_half(int V, dim3 grid, hipStream_t st, int8_t* own, const int8_t* other,
+                        const Int8Geom& g, const Int8Table& tb, u32 t, u32 color, u32 k0, u32 k1)
+{
+    if (V == 16)
+        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 16>), grid, dim3(256), 0, st, own, other, g,
+                           tb, t, color, k0, k1);
+    else if (V == 8)
+        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 8>), grid, dim3(256), 0, st, own, other, g,
+                           tb, t, color, k0, k1);
+    else
+        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 4>), grid, dim3(256), 0, st, own, other, g,
+                           tb, t, color, k0, k1);
+}
+
+static int int8_half_impl(int8_t* own, const int8_t* other, long long rows, int lh, int ly, int lz,
+                          int ndim, int heatbath, const u32* table, u32 t, u32 color, u32 k0,
+                          u32 k1, hipStream_t st)
+{
+    if (!own || !other || !table) return ising_fail(ISING_EINVAL, "int8_half: null pointer");
+    if (color > 1) return ising_fail(ISING_EINVAL, "int8_half: colour must be 0 or 1");
+    Int8Geom g;
+    int V;
+    int rc = make_geom(rows, lh, ly, lz, ndim, &g, &V);
+    if (rc) return rc;
+    Int8Table tb = {};
+    for (int i = 0; i <= 2 * ndim; ++i) tb.v[i] = table[i];
+    const dim3 grid((unsigned)((g.n_chunks + 255) / 256));
+    if (ndim == 2) {
+        if (heatbath) launch_half<2, 1>(V, grid, st, own, other, g, tb, t, color, k0, k1);
+        else launch_half<2, 0>(V, grid, st, own, other, g, tb, t, color, k0, k1);
+    } else {
+        if (heatbath) launch_half<3, 1>(V, grid, st, own, other, g, tb, t, color, k0, k1);
+        else launch_half<3, 0>(V, grid, st, own, other, g, tb, t, color, k0, k1);
+    }
+    ISING_LAUNCHED("int8_half_kernel");
+    return ISING_OK;
+}
+
+extern "C" {
+
+int ising_int8_half(int8_t* own, const int8_t* other, long long rows, int lh, int ly, int lz,
+                    int ndim, int heatbath, const uint32_t* table, uint32_t t, uint32_t color,
+                    uint32_t k0, uint32_t k1, void* stream)
+{
+    return int8_half_impl(own, other, rows, lh, ly, lz, ndim, heatbath, table, t, color, k0, k1,
+                          as_stream(stream));
+}
+
+int ising_int8_sweeps(int8_t* c0, int8_t* c1, long long rows, int lh, int ly, int lz, int ndim,
+                      int heatbath, const uint32_t* table, uint32_t t0, long long n_sweeps,
+                      uint32_t k0, uint32_t k1, void* stream)
+{
+    if (n_sweeps < 0) return ising_fail(ISING_EINVAL, "int8_sweeps: negative sweep count");
+    for (long long s = 0; s < n_sweeps; ++s) {
+        const u32 t = t0 + (u32)s;
+        int rc = int8_half_impl(c0, c1, rows, lh, ly, lz, ndim, heatbath, table, t, 0u, k0, k1,
+                                as_stream(stream));
+        if (rc) return rc;
+        rc = int8_half_impl(c1, c0, rows, lh, ly, lz, ndim, heatbath, table, t, 1u, k0, k1,
+                            as_stream(stream));
+        if (rc) return rc;
+    }
+    return ISING_OK;
+}
+
+int ising_int8_obs(const int8_t* c0, const int8_t* c1, long long rows, int lh, int ly, int lz,
+                   int ndim, unsigned long long* scratch2, long long* out2, void* stream)
+{
+    if (!c0 || !c1 || !scratch2 || !out2) return ising_fail(ISING_EINVAL, "int8_obs: null pointer");
+    Int8Geom g;
+    int V;
+    int rc = make_geom(rows, lh, ly, lz, ndim, &g, &V);
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, scratch2, 2);
+    const u64 blocks = (g.n_chunks + 255) / 256;
+    const dim3 grid((unsigned)(blocks < 2048 ? blocks : 2048));
+#define OBS_LAUNCH(ND, VV) \
+    hipLaunchKernelGGL((int8_obs_kernel<ND, VV>), grid, dim3(256), 0, st, c0, c1, g, scratch2)
+    if (ndim == 2) {
+        if (V == 16) OBS_LAUNCH(2, 16);
+        else if (V == 8) OBS_LAUNCH(2, 8);
+        else OBS_LAUNCH(2, 4);
+    } else {
+        if (V == 16) OBS_LAUNCH(3, 16);
+        else if (V == 8) OBS_LAUNCH(3, 8);
+        else OBS_LAUNCH(3, 4);
+    }
+#undef OBS_LAUNCH
+    const long long n_spins = 2 * rows * (long long)lh;
+    hipLaunchKernelGGL(obs_finalize_kernel, dim3(1), dim3(1), 0, st, scratch2, out2, n_spins, ndim);
+    ISING_LAUNCHED("int8_obs_kernel");
+    return ISING_OK;
+}
+
+}  // extern "C"

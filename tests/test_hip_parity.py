@@ -1,0 +1,197 @@
+"""The hip backend reproduces the oracle bit for bit (same Philox stream, same thresholds).
+
+Mirrors the reference's GPU identity suite (tests/test_backends.py:9-95 of the reference):
+int8 identity over shapes x algorithms, msc identity, set_beta with split sweeps, hot and cold
+starts; adds config-shaped cases, exact integer observables and every msc kernel mode.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle as CO
+from oracle import ising_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from pyising_amd.backends import hip_backend
+
+    return hip_backend
+
+
+@pytest.mark.parametrize("shape", [(16, 24), (8, 8, 16), (64, 32), (32, 64), (4, 6, 8)])
+@pytest.mark.parametrize("algorithm", ["metropolis", "heatbath"])
+def test_int8_bitwise_identical(hip, shape, algorithm):
+    lat = O.random_lattice(shape, 5)
+    beta = 0.42 if len(shape) == 2 else 0.22
+    ref = O.Int8Oracle(shape, beta, seed=2**40 + 7, algorithm=algorithm, init=lat)
+    sim = hip.HipInt8(shape, beta, seed=2**40 + 7, algorithm=algorithm, init=lat)
+    ref.sweep(6)
+    sim.sweep(6)
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    assert sim.observables_exact() == ref.observables_exact()
+    m, e = sim.observables()
+    assert m == pytest.approx(O.observables_exact(ref.get_lattice())[0] / lat.size)
+    assert e == pytest.approx(O.observables_exact(ref.get_lattice())[1] / lat.size)
+
+
+@pytest.mark.parametrize("mode,k", [(0, 8), (0, 2), (0, 16), (1, 0), (2, 0)])
+@pytest.mark.parametrize("shape", [(128, 128), (64, 384), (16, 1024), (130, 256)])
+def test_msc_bitwise_identical(hip, shape, mode, k):
+    lat = O.random_lattice(shape, 9)
+    ref = O.MscOracle(shape, 0.44, seed=11, init=lat)
+    cls = type("M", (hip.HipMultiSpin,), {"mode": mode, "tile_k": k or 8})
+    sim = cls(shape, 0.44, seed=11, init=lat)
+    ref.sweep(5)
+    sim.sweep(5)
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    assert sim.observables_exact() == ref.observables_exact()
+
+
+@pytest.mark.parametrize("variant", ["int8", "msc"])
+def test_set_beta_and_split_sweeps(hip, variant):
+    cls = hip.HipInt8 if variant == "int8" else hip.HipMultiSpin
+    shape = (128, 128)
+    lat = O.random_lattice(shape, 1)
+    a = cls(shape, 0.3, seed=3, init=lat)
+    b = cls(shape, 0.3, seed=3, init=lat)
+    a.sweep(2)
+    a.set_beta(0.5)
+    a.sweep(3)
+    b.sweep(2)
+    b.set_beta(0.5)
+    b.sweep(1)
+    b._half_sweep(0)
+    b._half_sweep(1)
+    b.t += 1
+    b.sweep(1)
+    np.testing.assert_array_equal(a.get_lattice(), b.get_lattice())
+    ref = (O.Int8Oracle if variant == "int8" else O.MscOracle)(shape, 0.3, seed=3, init=lat)
+    ref.sweep(2)
+    ref.set_beta(0.5)
+    ref.sweep(3)
+    np.testing.assert_array_equal(a.get_lattice(), ref.get_lattice())
+
+
+@pytest.mark.parametrize("variant", ["int8", "msc", "msc-eager32"])
+def test_hot_and_cold_starts(variant):
+    from pyising_amd import create
+
+    shape = (128, 128)
+    cold = create("hip", variant, shape=shape, beta=0.5, init="cold")
+    assert cold.observables() == pytest.approx((1.0, -2.0))
+    assert cold.observables_exact() == (shape[0] * shape[1], -2 * shape[0] * shape[1])
+    hot = create("hip", variant, shape=shape, beta=0.5, seed=1, init="hot")
+    np.testing.assert_array_equal(hot.get_lattice(), O.random_lattice(shape, 1))
+    m, e = hot.observables()
+    assert abs(m) < 0.05 and abs(e) < 0.05
+
+
+def test_cold_start_3d(hip):
+    sim = hip.HipInt8((8, 8, 16), 0.3, init="cold")
+    assert sim.observables_exact() == (1024, -3 * 1024)
+
+
+@pytest.mark.parametrize("variant,shape", [("int8", (16, 24)), ("int8", (6, 4, 8)),
+                                           ("msc", (64, 384))])
+def test_layout_roundtrip(hip, variant, shape):
+    cls = hip.HipInt8 if variant == "int8" else hip.HipMultiSpin
+    lat = O.random_lattice(shape, 123)
+    sim = cls(shape, 0.4, seed=1, init=lat)
+    np.testing.assert_array_equal(sim.get_lattice(), lat)
+    if variant == "msc":
+        c0, c1 = O.to_compressed(lat)
+        w0, w1 = sim.get_words()
+        np.testing.assert_array_equal(w0, O.pack_bits(c0))
+        np.testing.assert_array_equal(w1, O.pack_bits(c1))
+
+
+def test_philox_known_answers():
+    import ctypes as C
+
+    from pyising_amd import _lib
+
+    ctr = np.array([[0, 0, 0, 0], [0xFFFFFFFF] * 4,
+                    [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]], dtype=np.uint32)
+    key = np.array([[0, 0], [0xFFFFFFFF] * 2, [0xA4093822, 0x299F31D0]], dtype=np.uint32)
+    out = np.zeros_like(ctr)
+    _lib.check(_lib.lib().ising_philox_kat(ctr.ctypes.data, key.ctypes.data, out.ctypes.data, 3))
+    exp = np.array([[0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8],
+                    [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD],
+                    [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]], dtype=np.uint32)
+    np.testing.assert_array_equal(out, exp)
+    rng = np.random.default_rng(0)
+    ctr = rng.integers(0, 2**32, size=(4096, 4), dtype=np.uint32)
+    key = rng.integers(0, 2**32, size=(4096, 2), dtype=np.uint32)
+    out = np.zeros_like(ctr)
+    _lib.check(_lib.lib().ising_philox_kat(ctr.ctypes.data, key.ctypes.data, out.ctypes.data,
+                                           4096))
+    np.testing.assert_array_equal(out, CO.philox(ctr, key))
+    del C
+
+
+# ----------------------------------------------------------- config-shaped cases (C oracle) --
+def test_config_2d_int8_L1024_100_sweeps(hip):
+    """2D int8 L=1024, beta=0.4406868, seed=1, hot start, 100 sweeps, then (M, E)."""
+    shape, beta = (1024, 1024), 0.4406868
+    sim = hip.HipInt8(shape, beta, seed=1)
+    ref = CO.CInt8(shape, beta, seed=1)
+    sim.sweep(100)
+    ref.sweep(100)
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    assert sim.observables_exact() == ref.observables_exact()
+
+
+@pytest.mark.parametrize("algorithm", ["metropolis", "heatbath"])
+@pytest.mark.parametrize("K,init", [(0.221654626, "hot"), (0.20, "cold"), (0.24, "hot")])
+def test_config_3d_int8(hip, algorithm, K, init):
+    """3D int8, seed=7, cold and hot, (E, |M|) every 10 sweeps (L=32 here)."""
+    shape = (32, 32, 32)
+    sim = hip.HipInt8(shape, K, seed=7, algorithm=algorithm, init=init)
+    ref = CO.CInt8(shape, K, seed=7, algorithm=algorithm, init=init)
+    for _ in range(4):
+        sim.sweep(10)
+        ref.sweep(10)
+        assert sim.observables_exact() == ref.observables_exact()
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+
+
+def test_config_msc_L1024_tc(hip):
+    from pyising_amd import BETA_C_2D
+
+    shape = (1024, 1024)
+    sim = hip.HipMultiSpin(shape, BETA_C_2D, seed=1)
+    ref = CO.CMsc(shape, BETA_C_2D, seed=1)
+    sim.sweep(50)
+    ref.sweep(50)
+    w = sim.get_words()
+    np.testing.assert_array_equal(w[0], ref.w[0])
+    np.testing.assert_array_equal(w[1], ref.w[1])
+    assert sim.observables_exact() == ref.observables_exact()
+
+
+@pytest.mark.parametrize("T", [1.5, 2.269, 3.0])
+def test_config_int8_L2048_temperatures(hip, T):
+    shape = (2048, 2048)
+    sim = hip.HipInt8(shape, 1 / T, seed=1)
+    ref = CO.CInt8(shape, 1 / T, seed=1)
+    sim.sweep(5)
+    ref.sweep(5)
+    assert sim.observables_exact() == ref.observables_exact()
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+
+
+def test_config_msc_L16384_tc_few_sweeps(hip):
+    """The headline lattice itself: msc L=16384 at T_c, seed=1, hot start, 3 sweeps."""
+    from pyising_amd import BETA_C_2D
+
+    shape = (16384, 16384)
+    sim = hip.HipMultiSpin(shape, BETA_C_2D, seed=1)
+    ref = CO.CMsc(shape, BETA_C_2D, seed=1)
+    sim.sweep(3)
+    ref.sweep(3)
+    w0, w1 = sim.get_words()
+    assert np.array_equal(w0, ref.w[0]) and np.array_equal(w1, ref.w[1])
+    assert sim.observables_exact() == ref.observables_exact()
