@@ -1,0 +1,176 @@
+"""Benchmark: 2D multi-spin-coded Metropolis sweeps of an L=16384 torus at T_c on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One step = one full checkerboard sweep (two half-sweep launches) of the 16384^2 lattice
+(beta = 1/T_c exact, seed 1, hot start = numpy default_rng(1) draws), resident in HBM. Prints
+one JSON line: flips/ns (sweeps x spins / s / 1e9), the roofline of the dominant kernel (the
+msc half sweep) under the 0.375 B/update traffic model against the 8 TB/s HBM peak, the
+measured copy bandwidth and Philox ceiling, the batched-scan comparison when available, and
+the oracle timed on one host core as the CPU baseline.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+L_HEADLINE = 16384
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = 0.375
+PMC_FILE = ROOT / "profiles" / "msc_half_L16384_pmc.json"
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(seconds: float = 10.0) -> dict:
+    """The oracle's NumPy msc sweep on the same lattice (L=16384, T_c, seed 1, hot start),
+    one host core, timed over whole sweeps until ``seconds`` have passed."""
+    from oracle import ising_oracle as O
+    from pyising_amd.lattice import BETA_C_2D
+
+    sim = O.MscOracle((L_HEADLINE, L_HEADLINE), BETA_C_2D, seed=1)
+    sim.half_sweep(0)  # warm-up (allocations, page faults)
+    sweeps = 0
+    t0 = time.perf_counter()
+    while True:
+        sim.sweep(1)
+        sweeps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    rate = sweeps * L_HEADLINE**2 / el / 1e9
+    return {"value": round(rate, 4), "unit": "flips/ns", "cores": 1, "kind": "port",
+            "sample": f"oracle NumPy msc sweep, L={L_HEADLINE} T_c seed 1 hot start, "
+                      f"{sweeps} sweeps in {el:.1f} s on one core of {cpu_model()} "
+                      f"(os.cpu_count()={os.cpu_count()})"}
+
+
+def traffic_from_profile() -> int | None:
+    """HBM bytes per msc half-sweep launch from the committed rocprofv3 PMC summary."""
+    try:
+        d = json.loads(PMC_FILE.read_text())
+        return int(d["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ceilings", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path "
+              "is design-only in this build", file=sys.stderr)
+        return 2
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py is single-process; WORLD_SIZE > 1 is not supported", file=sys.stderr)
+        return 2
+
+    import ctypes as C
+
+    from pyising_amd import _lib
+    from pyising_amd.backends.hip_backend import HipMultiSpin
+    from pyising_amd.lattice import BETA_C_2D
+
+    lib = _lib.lib()
+    name = C.create_string_buffer(64)
+    n_cu = C.c_int()
+    total = C.c_longlong()
+    _lib.check(lib.ising_device_info(name, 64, C.byref(n_cu), C.byref(total)))
+
+    shape = (L_HEADLINE, L_HEADLINE)
+    n_spins = L_HEADLINE * L_HEADLINE
+    sim = HipMultiSpin(shape, BETA_C_2D, seed=1, init="hot")
+    sim.sweep(args.warmup)
+    sim.synchronize()
+    e0, e1 = _lib.Event(), _lib.Event()
+    _lib.device_sync()
+    t0 = time.perf_counter()
+    e0.record(sim.stream_ptr)
+    sim.sweep(args.steps)
+    e1.record(sim.stream_ptr)
+    _lib.device_sync()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_ms(e1)
+    flips_per_ns = args.steps * n_spins / wall / 1e9
+    launches = 2 * args.steps
+    kernel_s = gpu_ms * 1e-3 / launches  # average half-sweep launch, events on its stream
+    algo_bytes = BYTES_PER_UPDATE * n_spins / 2  # per half-sweep launch
+    achieved = algo_bytes / kernel_s / 1e9
+    m, e = sim.observables_exact()
+    stats = sim.philox_stats(2)
+
+    result = {
+        "metric": "spin flips/ns (2D msc Metropolis, L=16384, T=T_c)",
+        "value": round(flips_per_ns, 2),
+        "unit": "flips/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (hot start: numpy default_rng(1) draws)",
+        "config": {"workload": "2D multi-spin-coded Metropolis sweep, L=16384 torus, "
+                               "beta=1/T_c, seed=1, hot start; one step = one sweep",
+                   "L": L_HEADLINE, "beta": BETA_C_2D, "seed": 1, "kernel_mode": sim.mode,
+                   "tile_k": sim.tile_k},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile(),
+                     "algorithmic_bytes_per_launch": int(algo_bytes),
+                     "avg_launch_us": round(kernel_s * 1e6, 3)},
+        "device": {"arch": name.value.decode(), "cu": n_cu.value,
+                   "hbm_GiB": round(total.value / 2**30, 1)},
+        "final_M": m, "final_E": e,
+        "philox_calls_per_word": round(stats["calls_per_word"], 4),
+        "philox_slots_per_word": round(stats["slots_per_word"], 4),
+    }
+    del sim
+    if not args.no_ceilings:
+        from pyising_amd.probes import copy_bandwidth, philox_rate
+
+        bw = copy_bandwidth()
+        ph = philox_rate()
+        result["ceilings"] = {
+            "copy_bandwidth_GBs": round(bw, 1),
+            "pct_copy_bw": round(100 * flips_per_ns * BYTES_PER_UPDATE / bw, 2),
+            "philox_calls_per_ns": round(ph, 1),
+            "rng_ceiling_flips_per_ns": round(ph * 64 / stats["calls_per_word"], 1),
+            "memory_ceiling_flips_per_ns": round(bw / BYTES_PER_UPDATE, 1),
+        }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    code = main()
+    sys.stdout.flush()
+    os._exit(code)

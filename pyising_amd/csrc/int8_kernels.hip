@@ -33,7 +33,8 @@ struct Int8Geom {
     u32 lh;          // colour sites per row
     u32 ly, lz;      // rows = lz * ly (lz = 1 in 2D)
     u32 chunks_row;  // lh / V
-    u64 n_chunks;    // rows * lh / V
+    u32 n_chunks;    // rows * lh / V (< 2^32, checked on the host)
+    FastDiv div_chunks, div_ly;
 };
 
 template <int V>
@@ -111,11 +112,11 @@ __global__ __launch_bounds__(256) void int8_half_kernel(int8_t* __restrict__ own
 {
     constexpr int NWd = V / 4;
     const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
-    const u64 ci = (u64)blk * blockDim.x + threadIdx.x;
+    const u32 ci = blk * blockDim.x + threadIdx.x;
     if (ci >= g.n_chunks) return;
-    const u32 row = (u32)(ci / g.chunks_row);
-    const u32 j0 = (u32)(ci - (u64)row * g.chunks_row) * V;
-    const u32 z = row / g.ly;
+    const u32 row = fast_div(ci, g.div_chunks);
+    const u32 j0 = (ci - row * g.chunks_row) * V;
+    const u32 z = fast_div(row, g.div_ly);
     const u32 y = row - z * g.ly;
     const u32 p = (y + z + color) & 1u;
     const u64 rbase = (u64)row * g.lh;
@@ -212,11 +213,11 @@ __global__ __launch_bounds__(256) void int8_obs_kernel(const int8_t* __restrict_
 {
     constexpr int NWd = V / 4;
     u64 minus = 0, anti = 0;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 ci = (u64)blockIdx.x * blockDim.x + threadIdx.x; ci < g.n_chunks; ci += stride) {
-        const u32 row = (u32)(ci / g.chunks_row);
-        const u32 j0 = (u32)(ci - (u64)row * g.chunks_row) * V;
-        const u32 z = row / g.ly;
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 ci = blockIdx.x * blockDim.x + threadIdx.x; ci < g.n_chunks; ci += stride) {
+        const u32 row = fast_div(ci, g.div_chunks);
+        const u32 j0 = (ci - row * g.chunks_row) * V;
+        const u32 z = fast_div(row, g.div_ly);
         const u32 y = row - z * g.ly;
         const u32 p = (y + z) & 1u;  // colour 0
         const u64 rbase = (u64)row * g.lh;
@@ -305,10 +306,13 @@ static int make_geom(long long rows, int lh, int ly, int lz, int ndim, Int8Geom*
     g->lh = (u32)lh;
     g->ly = (u32)ly;
     g->lz = (u32)lz;
+    const u64 n_chunks = (u64)rows * (u64)(lh / v);
+    if (n_chunks + 256 > 0xFFFFFFFFull)
+        return ising_fail(ISING_ERANGE, "int8: lattice too large (rows * Lx/2 / V must be < 2^32)");
     g->chunks_row = (u32)(lh / v);
-    g->n_chunks = (u64)rows * (u64)(lh / v);
-    if (g->n_chunks / 256 + 1 > 0x7FFFFFFFull)
-        return ising_fail(ISING_ERANGE, "int8: lattice too large for one launch");
+    g->n_chunks = (u32)n_chunks;
+    g->div_chunks = make_fastdiv(g->chunks_row);
+    g->div_ly = make_fastdiv(g->ly);
     *V = v;
     return ISING_OK;
 }

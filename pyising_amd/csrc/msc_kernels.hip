@@ -30,8 +30,9 @@ struct MscArgs {
     u64* own;
     const u64* other;
     u32 rows, nw;
-    u64 n_words;
+    u32 n_words;  // rows * nw (< 2^32, checked on the host)
     u32 t, color, k0, k1, thr4, thr8;
+    FastDiv div_nw;
 };
 
 struct MscWord {
@@ -39,10 +40,10 @@ struct MscWord {
 };
 
 // Neighbour classification of word w (global index), loads through the cache hierarchy.
-__device__ __forceinline__ MscWord msc_classify(const MscArgs& a, u64 w)
+__device__ __forceinline__ MscWord msc_classify(const MscArgs& a, u32 w)
 {
-    const u32 y = (u32)(w / a.nw);
-    const u32 x = (u32)(w - (u64)y * a.nw);
+    const u32 y = fast_div(w, a.div_nw);
+    const u32 x = w - y * a.nw;
     const u64 rbase = (u64)y * a.nw;
     const u32 p = (y + a.color) & 1u;
     const u64 o = a.own[w];
@@ -90,14 +91,14 @@ template <bool LAZY>
 __global__ __launch_bounds__(256) void msc_half_lockstep(MscArgs a, unsigned long long* stats)
 {
     const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
-    const u64 w = (u64)blk * blockDim.x + threadIdx.x;
+    const u32 w = blk * blockDim.x + threadIdx.x;
     const bool valid = w < a.n_words;
     u32 calls = 0;
     if (valid) {
         const MscWord m = msc_classify(a, w);
         u64 acc = 0, und = m.a0 | m.a1;
         for (u32 lp = 0; lp < 16u && (!LAZY || und); ++lp) {
-            const PhiloxOut r = philox4x32_10((u32)w, a.t, a.color, 1u + lp, a.k0, a.k1);
+            const PhiloxOut r = philox4x32_10(w, a.t, a.color, 1u + lp, a.k0, a.k1);
             ++calls;
             msc_lazy_pair(r, lp, a.thr4, a.thr8, m.a0, m.a1, acc, und);
         }
@@ -138,14 +139,14 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u32 n_tiles,
     const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
     const u32 tile = blk * 4u + wave;
     const bool tile_ok = tile < n_tiles;
-    const u64 base = (u64)tile * T;
+    const u32 base = tile * T;
 
     // phase 1: classify every word of the tile (coalesced: word base + i * 64 + lane)
     if (tile_ok) {
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             const u32 s = (u32)i * 64u + lane;
-            const u64 w = base + s;
+            const u32 w = base + s;
             if (w < a.n_words) {
                 const MscWord m = msc_classify(a, w);
                 L0[s] = m.a0;
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u32 n_tiles,
         u32 cur = 0, lp = 0;
         u64 a0 = 0, a1 = 0, acc = 0, und = 0;
         bool live = true;
-        const u32 wbase = (u32)base;  // Philox word counter, truncated to 32 bits
+        const u32 wbase = base;  // Philox word counter
         while (true) {
             bool need = live && (und == 0);
             u64 m = __ballot(need);
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u32 n_tiles,
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             const u32 s = (u32)i * 64u + lane;
-            const u64 w = base + s;
+            const u32 w = base + s;
             if (w < a.n_words) a.own[w] = LP[s];
         }
     }
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u32 n_tiles,
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, 64);
         if (lane == 0) {
-            const u64 nw = (a.n_words - base < T) ? (a.n_words - base) : T;
+            const u32 nw = (a.n_words - base < T) ? (a.n_words - base) : T;
             atomicAdd(stats + 0, (unsigned long long)tot);
             atomicAdd(stats + 1, (unsigned long long)iters * 64ull);
             atomicAdd(stats + 2, (unsigned long long)nw);
@@ -231,13 +232,14 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u32 n_tiles,
 // (src/pyising/backends/observables_torch.py:85-111).
 __global__ __launch_bounds__(256) void msc_obs_kernel(const u64* __restrict__ w0,
                                                       const u64* __restrict__ w1, u32 rows, u32 nw,
-                                                      u64 n_words, unsigned long long* acc)
+                                                      u32 n_words, FastDiv div_nw,
+                                                      unsigned long long* acc)
 {
     u64 ones = 0, anti = 0;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 w = (u64)blockIdx.x * blockDim.x + threadIdx.x; w < n_words; w += stride) {
-        const u32 y = (u32)(w / nw);
-        const u32 x = (u32)(w - (u64)y * nw);
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 w = blockIdx.x * blockDim.x + threadIdx.x; w < n_words; w += stride) {
+        const u32 y = fast_div(w, div_nw);
+        const u32 x = w - y * nw;
         const u64 rbase = (u64)y * nw;
         const u32 p = y & 1u;
         const u64 s = w0[w], c = w1[w];
@@ -285,7 +287,8 @@ static int msc_check(const void* own, const void* other, long long rows, int nw)
 {
     if (!own || !other) return ising_fail(ISING_EINVAL, "msc: null pointer");
     if (rows < 2 || (rows & 1) || nw < 1) return ising_fail(ISING_EINVAL, "msc: bad geometry");
-    if (rows > 0x7FFFFFFFll) return ising_fail(ISING_ERANGE, "msc: too many rows");
+    if ((unsigned long long)rows * (unsigned long long)nw + 65536ull > 0xFFFFFFFFull)
+        return ising_fail(ISING_ERANGE, "msc: rows * Lx/128 must be < 2^32 words per colour");
     return ISING_OK;
 }
 
@@ -315,7 +318,8 @@ static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32
     a.other = other;
     a.rows = (u32)rows;
     a.nw = (u32)nw;
-    a.n_words = (u64)rows * (u64)nw;
+    a.n_words = (u32)(rows * nw);
+    a.div_nw = make_fastdiv((u32)nw);
     a.t = t;
     a.color = color;
     a.k0 = k0;
@@ -416,7 +420,7 @@ int ising_msc_obs(const uint64_t* w0, const uint64_t* w1, long long rows, int nw
     hipLaunchKernelGGL(msc_zero2, dim3(1), dim3(64), 0, st, scratch2);
     const u64 blocks = (n_words + 255) / 256;
     hipLaunchKernelGGL(msc_obs_kernel, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(256), 0,
-                       st, w0, w1, (u32)rows, (u32)nw, n_words, scratch2);
+                       st, w0, w1, (u32)rows, (u32)nw, (u32)n_words, make_fastdiv((u32)nw), scratch2);
     hipLaunchKernelGGL(msc_obs_finalize, dim3(1), dim3(1), 0, st, scratch2, out2,
                        (long long)n_words * 128);
     ISING_LAUNCHED("msc_obs_kernel");
