@@ -1,0 +1,150 @@
+/*
+ * ising_hip.h -- C ABI of libising_hip.so, the gfx950 (MI355X) kernels of the pyising `hip`
+ * backend.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a negative status (ISING_EINVAL = -1 bad
+ *     argument, ISING_EHIP = -2 HIP runtime failure, ISING_ERANGE = -3 size beyond what the
+ *     entry point supports); ising_last_error() gives the message of the calling thread's last
+ *     failure. No exceptions cross the ABI.
+ *   - Pointers named d_* / spin arrays / scratch / out are device pointers owned by the
+ *     caller; `table` and the Philox known-answer arrays are host pointers.
+ *   - `stream` is a hipStream_t (NULL = the null stream). Launches are stream-ordered and
+ *     asynchronous unless stated otherwise; one host thread per simulation.
+ *   - Layouts follow the reference (src/pyising/lattice.py:1-14): colour c is a row-major
+ *     (rows, Lx/2) int8 array (+1 = 0x01, -1 = 0xFF), rows = Ly (2D) or Lz*Ly (3D); the packed
+ *     msc layout is (rows, Lx/128) uint64 with bit b of word x = colour site 64x + b, 1 = +1.
+ *   - Philox key (k0, k1) = (low, high) 32 bits of the 64-bit seed (src/pyising/philox.py:20-23);
+ *     `t` is the sweep counter (counter word 1), colour is counter word 2.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   IsingSimulation._half_sweep          src/pyising/backends/base.py:93-103
+ *   CupyInt8._half_sweep / int8_half     src/pyising/backends/cupy_backend.py:68-72,
+ *                                        src/pyising/backends/cuda/ising.cu:35-87
+ *   CupyMultiSpin._half_sweep / msc_half src/pyising/backends/cupy_backend.py:129-135,
+ *                                        src/pyising/backends/cuda/ising.cu:102-150
+ *   IsingSimulation.sweep                src/pyising/backends/base.py:85-90
+ *   observables_int8 / observables_packed src/pyising/backends/observables_torch.py:46-111
+ *   to/from_compressed, pack/unpack_bits src/pyising/lattice.py:49-90
+ *   philox_calls_per_word (stats buffer) src/pyising/backends/cupy_backend.py:137-144
+ *   philox4x32_10                        src/pyising/backends/cuda/ising.cu:10-21
+ *   copy_bandwidth, philox_rate probes   bench/roofline.py:24-78
+ */
+#ifndef ISING_HIP_H
+#define ISING_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status, runtime ---- */
+/* Message of the calling thread's last failure (empty if none). No reference counterpart. */
+const char* ising_last_error(void);
+/* Number of visible GPUs; the reference gates GPU entries on torch.cuda.is_available()
+ * (src/pyising/backends/_gpu.py:39-46). */
+int ising_device_count(int* n);
+/* Architecture name, compute-unit count and HBM bytes of the current device
+ * (bench/run.py:82-106 machine_info). */
+int ising_device_info(char* name, int name_len, int* n_cu, long long* total_bytes);
+/* Device memory and copies: the reference allocates through CuPy (cupy_backend.py:62,106,150). */
+int ising_malloc(void** ptr, size_t bytes);
+int ising_free(void* ptr);
+int ising_mem_info(size_t* free_bytes, size_t* total_bytes);
+int ising_memcpy_h2d(void* d_dst, const void* src, size_t bytes, void* stream); /* synchronous */
+int ising_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream); /* synchronous */
+int ising_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, void* stream);
+/* Streams (default priority, non-blocking) and events; synchronize() of base.py:130-131. */
+int ising_stream_create(void** stream);
+int ising_stream_destroy(void* stream);
+int ising_stream_sync(void* stream);
+int ising_device_sync(void);
+int ising_event_create(void** ev);
+int ising_event_destroy(void* ev);
+int ising_event_record(void* ev, void* stream);
+int ising_event_query(void* ev); /* 1 = complete, 0 = pending, < 0 error */
+int ising_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ----------------------------------------------------------------- layouts ---- */
+/* Full lattice (rows, lx) bytes -> two int8 colour arrays (rows, lx/2); any byte > 0 is +1,
+ * anything else -1 (so raw {0,1} hot-start draws are accepted). ly = rows per z-plane
+ * (ly = rows in 2D). Replaces to_compressed, src/pyising/lattice.py:49-60. lx % 8 == 0. */
+int ising_split_int8(const void* d_full, int8_t* d_c0, int8_t* d_c1, long long rows, int ly, int lx,
+                     void* stream);
+/* Inverse: from_compressed, src/pyising/lattice.py:63-72. */
+int ising_merge_int8(const int8_t* d_c0, const int8_t* d_c1, void* d_full, long long rows, int ly,
+                     int lx, void* stream);
+/* Full 2D lattice -> two packed colour arrays (rows, lx/128): to_compressed + pack_bits,
+ * src/pyising/lattice.py:49-82. lx % 128 == 0. */
+int ising_split_pack(const void* d_full, uint64_t* d_w0, uint64_t* d_w1, long long rows, int lx,
+                     void* stream);
+/* Inverse: unpack_bits + from_compressed, src/pyising/lattice.py:63-72,85-90. */
+int ising_unpack_merge(const uint64_t* d_w0, const uint64_t* d_w1, void* d_full, long long rows,
+                       int lx, void* stream);
+/* Fill n uint64 words with `value` (cold starts, counters): _fill_cold of
+ * src/pyising/backends/base.py:119-120, cupy_backend.py:154-156. */
+int ising_fill_u64(uint64_t* d_dst, long long n, uint64_t value, void* stream);
+
+/* ----------------------------------------------------------- int8 checkerboard ---- */
+/* One half sweep of colour `color`: own (rows, lh) updated in place from other.
+ * rows = ly * lz (lz = 1 in 2D); ndim 2 or 3; heatbath 0/1; table = host array of 2*ndim+1
+ * uint32 thresholds (src/pyising/lattice.py:128-142). lh % 4 == 0.
+ * Replaces CupyInt8._half_sweep (cupy_backend.py:68-72) / int8_half (ising.cu:35-87). */
+int ising_int8_half(int8_t* d_own, const int8_t* d_other, long long rows, int lh, int ly, int lz,
+                    int ndim, int heatbath, const uint32_t* table, uint32_t t, uint32_t color,
+                    uint32_t k0, uint32_t k1, void* stream);
+/* n_sweeps full sweeps (colour 0 then 1, t = t0, t0+1, ...): IsingSimulation.sweep
+ * (base.py:85-90) with all launches issued from C. */
+int ising_int8_sweeps(int8_t* d_c0, int8_t* d_c1, long long rows, int lh, int ly, int lz, int ndim,
+                      int heatbath, const uint32_t* table, uint32_t t0, long long n_sweeps,
+                      uint32_t k0, uint32_t k1, void* stream);
+/* Exact (M, E) into out2[0..1] (device int64); scratch2 = 2 device uint64 (any contents).
+ * observables_int8, src/pyising/backends/observables_torch.py:46-69. */
+int ising_int8_obs(const int8_t* d_c0, const int8_t* d_c1, long long rows, int lh, int ly, int lz,
+                   int ndim, unsigned long long* d_scratch2, long long* d_out2, void* stream);
+
+/* ------------------------------------------------------ multi-spin coding (2D) ---- */
+/* Process default msc kernel: mode 0 tile queue (k in {2,4,8,16}), 1 lockstep, 2 eager
+ * (all 16 calls; the msc-eager32 ablation of cupy_backend.py:176-180), 3 streaming queue
+ * (k = 64-word groups per wave, 1..64). No reference counterpart (kernel selection). */
+int ising_msc_config(int mode, int k);
+/* One half sweep with the process default kernel. thr4 = table[3], thr8 = table[4].
+ * stats (optional device uint64[3], accumulated): Philox calls, lane-slots executed, words.
+ * Replaces CupyMultiSpin._half_sweep (cupy_backend.py:129-135) / msc_half (ising.cu:102-150)
+ * and the stats buffer of cupy_backend.py:137-144. */
+int ising_msc_half(uint64_t* d_own, const uint64_t* d_other, long long rows, int nw, uint32_t t,
+                   uint32_t color, uint32_t k0, uint32_t k1, uint32_t thr4, uint32_t thr8,
+                   unsigned long long* d_stats, void* stream);
+/* Same with an explicit kernel mode and k (mode < 0: process default). */
+int ising_msc_half_ex(uint64_t* d_own, const uint64_t* d_other, long long rows, int nw, uint32_t t,
+                      uint32_t color, uint32_t k0, uint32_t k1, uint32_t thr4, uint32_t thr8,
+                      int mode, int k, unsigned long long* d_stats, void* stream);
+/* n_sweeps full sweeps: IsingSimulation.sweep (base.py:85-90). */
+int ising_msc_sweeps(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw, uint32_t t0,
+                     long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4, uint32_t thr8,
+                     unsigned long long* d_stats, void* stream);
+int ising_msc_sweeps_ex(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw, uint32_t t0,
+                        long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4,
+                        uint32_t thr8, int mode, int k, unsigned long long* d_stats,
+                        void* stream);
+/* Exact (M, E) of a packed lattice: observables_packed, observables_torch.py:85-111. */
+int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, int nw,
+                  unsigned long long* d_scratch2, long long* d_out2, void* stream);
+
+/* ------------------------------------------------------------------ probes ---- */
+/* Philox4x32-10 of n (counter[4], key[2]) pairs, host arrays in and out: the known-answer
+ * path of philox4x32_10 (ising.cu:10-21; tests/test_philox.py:7-18). Synchronous. */
+int ising_philox_kat(const uint32_t* ctr, const uint32_t* key, uint32_t* out, int n);
+/* Throughput probe: blocks x 256 threads x `calls` Philox calls (bench/roofline.py:24-36). */
+int ising_philox_rate(uint32_t* d_out, int blocks, int calls, uint32_t k0, uint32_t k1,
+                      void* stream);
+/* Streaming copy kernel, bytes % 16 == 0 (bench/roofline.py:39-56 copy bandwidth). */
+int ising_copy(void* d_dst, const void* d_src, size_t bytes, int blocks, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISING_HIP_H */

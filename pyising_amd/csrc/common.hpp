@@ -50,14 +50,36 @@ struct PhiloxOut {
     u32 x, y, z, w;
 };
 
+// a ^ b ^ c in one VALU op: gfx950's v_bitop3_b32 with truth table 0x96.
+__device__ __forceinline__ u32 xor3(u32 a, u32 b, u32 c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// On CDNA the 32x32->64 multiply (v_mad_u64_u32) issues at the full 64-bit rate, so the
+// round's XORs dominate; rounds 1-9 use one v_bitop3 per output word instead of two XORs.
+// Round 0 keeps plain XORs so that its wave-uniform half (counter words 1 and 2 are the sweep
+// and the colour) stays on the scalar unit.
 __device__ __forceinline__ PhiloxOut philox4x32_10(u32 c0, u32 c1, u32 c2, u32 c3, u32 k0, u32 k1)
 {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
+    {
         const u64 p0 = (u64)c0 * PHILOX_M0;
         const u64 p1 = (u64)c2 * PHILOX_M1;
         const u32 n0 = (u32)(p1 >> 32) ^ c1 ^ k0;
         const u32 n2 = (u32)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = (u32)p1;
+        c2 = n2;
+        c3 = (u32)p0;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+#pragma unroll
+    for (int r = 1; r < 10; ++r) {
+        const u64 p0 = (u64)c0 * PHILOX_M0;
+        const u64 p1 = (u64)c2 * PHILOX_M1;
+        const u32 n0 = xor3((u32)(p1 >> 32), c1, k0);
+        const u32 n2 = xor3((u32)(p0 >> 32), c3, k1);
         c0 = n0;
         c1 = (u32)p1;
         c2 = n2;

@@ -226,6 +226,152 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u32 n_tiles,
     }
 }
 
+// Two lazy levels with per-lane level index lp (the work-queue kernels): the threshold bit of
+// level b is a sign-extended one-bit field, 0 or ~0, of the scalar threshold.
+__device__ __forceinline__ void msc_lazy_pair_lane(const PhiloxOut& r, u32 lp, u32 thr4, u32 thr8,
+                                                   u64 a0, u64 a1, u64& acc, u64& und)
+{
+    const u32 b0 = 31u - 2u * lp;
+    const u32 lvs[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int lev = 0; lev < 2; ++lev) {
+        const u32 b = b0 - (u32)lev;
+        const u32 t4 = (u32)__builtin_amdgcn_sbfe((int)thr4, b, 1);
+        const u32 t8 = (u32)__builtin_amdgcn_sbfe((int)thr8, b, 1);
+        u32 ac[2] = {(u32)acc, (u32)(acc >> 32)};
+        u32 un[2] = {(u32)und, (u32)(und >> 32)};
+        const u32 x0[2] = {(u32)a0, (u32)(a0 >> 32)};
+        const u32 x1[2] = {(u32)a1, (u32)(a1 >> 32)};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const u32 lv = lvs[2 * lev + h];
+            const u32 tm = (x1[h] & t4) | (x0[h] & t8);
+            ac[h] |= un[h] & ~lv & tm;
+            un[h] &= ~(lv ^ tm);
+        }
+        acc = (u64)ac[0] | ((u64)ac[1] << 32);
+        und = (u64)un[0] | ((u64)un[1] << 32);
+    }
+}
+
+// Streaming work queue. Each wave owns `groups` consecutive groups of 64 words. A 128-slot
+// ring in LDS holds classified, unclaimed words (a0, a1, own ^ a2); slot counter c maps to
+// word wbase + c, so no index is stored. A lane claims the next slot when its word is
+// decided (ballot + mbcnt rank), copies the slot into registers (the slot is free at once),
+// and stores the finished word straight to `own`. The ring is topped up with the next group
+// (coalesced loads by all 64 lanes) whenever it holds <= 64 words, so lanes only idle in the
+// final drain of the wave's range. 3 KB of LDS per wave keeps 8 waves per SIMD resident.
+__global__ __launch_bounds__(256) void msc_half_stream(MscArgs a, u32 n_groups, u32 groups,
+                                                      unsigned long long* stats)
+{
+    __shared__ u64 ring[4][3][128];
+    const u32 wave = threadIdx.x >> 6;
+    const u32 lane = threadIdx.x & 63u;
+    u64* R0 = ring[wave][0];
+    u64* R1 = ring[wave][1];
+    u64* RP = ring[wave][2];
+    const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
+    const u32 wid = blk * 4u + wave;
+    const u32 g_begin = wid * groups;
+    if (g_begin >= n_groups) return;  // whole wave exits; no block barriers below
+    const u32 g_end = min(g_begin + groups, n_groups);
+    const u32 wbase = g_begin * 64u;
+
+    u32 gnext = g_begin;
+    u32 head = 0, tail = 0;  // wave-uniform ring counters
+    bool has = false;
+    u32 word = 0, lp = 0;
+    u64 a0 = 0, a1 = 0, pw = 0, acc = 0, und = 0;
+    u64 iters = 0, calls = 0;
+
+    while (true) {
+        // claim words for idle lanes, topping the ring up when it runs dry
+        while (true) {
+            const bool need = !has;
+            const u64 m = __ballot(need);
+            if (m == 0) break;
+            if (tail == head) {
+                if (gnext >= g_end) break;
+                const u32 w = gnext * 64u + lane;
+                const u32 s = tail & 127u;
+                if (w < a.n_words) {
+                    const MscWord cw = msc_classify(a, w);
+                    R0[s] = cw.a0;
+                    R1[s] = cw.a1;
+                    RP[s] = cw.o ^ cw.a2;
+                } else {
+                    R0[s] = 0;
+                    R1[s] = 0;
+                }
+                __builtin_amdgcn_wave_barrier();
+                tail += 64u;
+                ++gnext;
+            }
+            const u32 avail = tail - head;
+            const u32 rank = lane_rank(m);
+            if (need && rank < avail) {
+                const u32 c = head + rank;
+                const u32 s = c & 127u;
+                word = wbase + c;
+                a0 = R0[s];
+                a1 = R1[s];
+                pw = RP[s];
+                und = a0 | a1;
+                acc = 0;
+                lp = 0;
+                has = true;
+                if (und == 0) {  // decided without random numbers
+                    if (word < a.n_words) a.own[word] = pw;
+                    has = false;
+                }
+            }
+            const u32 taken = (u32)__popcll(m);
+            head += taken < avail ? taken : avail;
+        }
+        // keep one group queued ahead
+        if (tail - head <= 64u && gnext < g_end) {
+            const u32 w = gnext * 64u + lane;
+            const u32 s = tail & 127u;
+            if (w < a.n_words) {
+                const MscWord cw = msc_classify(a, w);
+                R0[s] = cw.a0;
+                R1[s] = cw.a1;
+                RP[s] = cw.o ^ cw.a2;
+            } else {
+                R0[s] = 0;
+                R1[s] = 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            tail += 64u;
+            ++gnext;
+        }
+        if (__ballot(has) == 0) break;
+        ++iters;
+        if (has) {
+            const PhiloxOut r = philox4x32_10(word, a.t, a.color, 1u + lp, a.k0, a.k1);
+            ++calls;
+            msc_lazy_pair_lane(r, lp, a.thr4, a.thr8, a0, a1, acc, und);
+            ++lp;
+            if (lp == 16u) und = 0;  // all 32 bits compared: u == thr, not accepted
+            if (und == 0) {
+                a.own[word] = pw ^ acc;
+                has = false;
+            }
+        }
+    }
+    if (stats) {
+        u64 tot = calls;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, 64);
+        if (lane == 0) {
+            const u32 wend = min(g_end * 64u, a.n_words);
+            atomicAdd(stats + 0, (unsigned long long)tot);
+            atomicAdd(stats + 1, (unsigned long long)iters * 64ull);
+            atomicAdd(stats + 2, (unsigned long long)(wend - wbase));
+        }
+    }
+}
+
 // ------------------------------------------------------------------------ observables ----
 // Colour-0 word s with other-colour neighbours (c, hz, up, dn): ones += popc(s) + popc(c),
 // anti += popc(s ^ nb) over the four; M = 2 ones - N, E = 2 anti - 2 N
@@ -278,8 +424,9 @@ __global__ void msc_zero2(unsigned long long* p)
 }
 
 // ---------------------------------------------------------------------- host launchers ----
-// Kernel modes: 0 = work queue (tile of 64*K words per wave, K in {2,4,8,16}),
-// 1 = lockstep (one word per lane), 2 = eager lockstep (all 16 calls, ablation).
+// Kernel modes: 0 = tile work queue (tile of 64*K words per wave, K in {2,4,8,16}),
+// 1 = lockstep (one word per lane), 2 = eager lockstep (all 16 calls, ablation),
+// 3 = streaming work queue (K = groups of 64 words per wave, 1..64).
 static int g_msc_mode = 0;
 static int g_msc_k = 8;
 
@@ -295,9 +442,12 @@ static int msc_check(const void* own, const void* other, long long rows, int nw)
 static int msc_mode_ok(int mode, int k)
 {
     if (mode < 0) return ISING_OK;  // process default
-    if (mode > 2) return ising_fail(ISING_EINVAL, "msc: mode must be 0 (queue), 1 (lockstep) or 2 (eager)");
+    if (mode > 3)
+        return ising_fail(ISING_EINVAL, "msc: mode must be 0 (tile queue), 1 (lockstep), 2 (eager) or 3 (stream)");
     if (mode == 0 && k != 2 && k != 4 && k != 8 && k != 16)
         return ising_fail(ISING_EINVAL, "msc: tile K must be 2, 4, 8 or 16");
+    if (mode == 3 && (k < 1 || k > 64))
+        return ising_fail(ISING_EINVAL, "msc: stream groups per wave must be in 1..64");
     return ISING_OK;
 }
 
@@ -326,7 +476,12 @@ static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32
     a.k1 = k1;
     a.thr4 = thr4;
     a.thr8 = thr8;
-    if (mode != 0) {
+    if (mode == 3) {
+        const u32 n_groups = (a.n_words + 63u) / 64u;
+        const u32 waves = (n_groups + (u32)k - 1u) / (u32)k;
+        const u32 blocks = (waves + 3u) / 4u;
+        hipLaunchKernelGGL(msc_half_stream, dim3(blocks), dim3(256), 0, st, a, n_groups, (u32)k, stats);
+    } else if (mode != 0) {
         const u64 blocks = (a.n_words + 255) / 256;
         if (blocks > 0x7FFFFFFFull) return ising_fail(ISING_ERANGE, "msc: lattice too large");
         if (mode == 1)
@@ -373,7 +528,7 @@ int ising_msc_config(int mode, int k)
     int rc = msc_mode_ok(mode, k);
     if (rc) return rc;
     g_msc_mode = mode;
-    g_msc_k = (mode == 0) ? k : g_msc_k;
+    g_msc_k = (mode == 0 || mode == 3) ? k : g_msc_k;
     return ISING_OK;
 }
 

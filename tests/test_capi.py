@@ -1,0 +1,69 @@
+"""The C-ABI library loads without a GPU and exports every symbol include/ising_hip.h declares,
+with the argument counts the ctypes layer uses (no compute calls here)."""
+
+import re
+from pathlib import Path
+
+import pytest
+
+from pyising_amd import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "ising_hip.h"
+
+
+def declared() -> dict[str, int]:
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(?:int|const char\*)\s+(ising_\w+)\s*\(([^;]*?)\)\s*;", text, re.S):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_header_parses():
+    decl = declared()
+    assert len(decl) >= 30
+    assert "ising_msc_half" in decl and "ising_int8_half" in decl
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load_library()
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_ctypes_signatures_match_header():
+    decl = declared()
+    for name, nargs in decl.items():
+        if name == "ising_last_error":
+            assert nargs == 0
+            continue
+        assert name in _lib.SIGNATURES, f"{name} declared but not bound"
+        assert len(_lib.SIGNATURES[name]) == nargs, name
+    assert set(_lib.SIGNATURES) <= set(decl)
+
+
+def test_errors_without_gpu_are_loud():
+    if _lib.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(_lib.HipError):
+        _lib.lib()
+    from pyising_amd.backends.hip_backend import HipMultiSpin
+
+    with pytest.raises(_lib.HipError):
+        HipMultiSpin((128, 128), 0.4)
+
+
+def test_argument_validation_needs_no_gpu():
+    """Bad arguments are rejected by the C layer before any HIP call."""
+    lib = _lib.load_library()
+    rc = lib.ising_int8_half(None, None, 16, 12, 16, 1, 2, 0, None, 0, 0, 0, 0, None)
+    assert rc == -1
+    assert b"null" in lib.ising_last_error()
+    rc = lib.ising_msc_config(7, 8)
+    assert rc == -1 and b"mode" in lib.ising_last_error()
+    rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 3, 0, None, None)
+    assert rc == -1 and b"groups" in lib.ising_last_error()
+    rc = lib.ising_split_pack(1, 1, 1, 4, 100, None)
+    assert rc == -1 and b"128" in lib.ising_last_error()
