@@ -106,9 +106,10 @@ int ising_int8_obs(const int8_t* d_c0, const int8_t* d_c1, long long rows, int l
                    int ndim, unsigned long long* d_scratch2, long long* d_out2, void* stream);
 
 /* ------------------------------------------------------ multi-spin coding (2D) ---- */
-/* Process default msc kernel: mode 0 tile queue (k in {2,4,8,16}), 1 lockstep, 2 eager
- * (all 16 calls; the msc-eager32 ablation of cupy_backend.py:176-180), 3 streaming queue
- * (k = 64-word groups per wave, 1..64). No reference counterpart (kernel selection). */
+/* Process default msc kernel: mode 0 lockstep + survivor compaction (k = 10 * cap + R, cap in
+ * 1..15 lockstep levels, R in {1,2,4} x 256 words per workgroup; default k = 42), 1 lockstep,
+ * 2 eager (all 16 calls; the msc-eager32 ablation of cupy_backend.py:176-180). All modes give
+ * identical words. No reference counterpart (kernel selection). */
 int ising_msc_config(int mode, int k);
 /* One half sweep with the process default kernel. thr4 = table[3], thr8 = table[4].
  * stats (optional device uint64[3], accumulated): Philox calls, lane-slots executed, words.

@@ -23,7 +23,7 @@ _ONES_INT8 = 0x0101010101010101
 _ALL_ONES = 0xFFFFFFFFFFFFFFFF
 
 #: msc kernel modes of ising_msc_sweeps_ex
-MSC_QUEUE, MSC_LOCKSTEP, MSC_EAGER = 0, 1, 2
+MSC_COMPACT, MSC_LOCKSTEP, MSC_EAGER = 0, 1, 2
 
 
 def _full_lattice(lat: np.ndarray, shape) -> np.ndarray:
@@ -132,9 +132,9 @@ class HipMultiSpin(_HipBase):
     layout_multiple = 128
     stream = "msc"
     bytes_per_update = 3.0 / 8.0
-    #: kernel mode and tile size K (words per lane) for the work-queue kernel
-    mode = MSC_QUEUE
-    tile_k = 8
+    #: kernel mode and its parameter (compaction: 10 * cap + R, see ising_msc_config)
+    mode = MSC_COMPACT
+    tile_k = 42
 
     def _setup(self) -> None:
         self._hip_setup()

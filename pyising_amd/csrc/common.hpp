@@ -122,6 +122,12 @@ __device__ __forceinline__ u32 fast_div(u32 n, const FastDiv& f)
     return (t + ((n - t) >> f.s1)) >> f.s2;
 }
 
+// Number of set bits of a wave-wide mask below the calling lane (wave64).
+__device__ __forceinline__ u32 lane_rank(u64 mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
+}
+
 // --------------------------------------------------------------------- launch helpers ----
 // MI355X dispatches workgroup i to XCD i % 8. For stencil kernels whose neighbouring blocks
 // share halo rows, remap so that each XCD walks one contiguous band of blocks: rows then

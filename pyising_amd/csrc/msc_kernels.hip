@@ -11,19 +11,23 @@
 // >= 2 anti-aligned neighbours flip (a2); lanes with exactly one (a1) or none (a0) flip iff
 // u < thr4 (thr8), where u's bit 31 - k is bit b of random level k, levels 2l and 2l + 1 coming
 // from Philox4x32-10((u32)w, t, colour, 1 + l; k0, k1) as r0 | r1 << 32 and r2 | r3 << 32
-// (src/pyising/philox.py:67-78). The comparison is decided MSB-first and stops as soon as no
-// lane of the word is undecided, so the number of Philox calls per word is data dependent
-// (about 3.8 on average at T_c, up to 16).
+// (src/pyising/philox.py:67-78). The comparison is decided MSB-first and stops once no lane of
+// the word is undecided, so the number of Philox calls per word is data dependent (about 3.8
+// on average at T_c, up to 16). Any lane may evaluate any (word, level) call: the stream is
+// counter-based, which is what lets the kernels below regroup work.
 //
-// Two kernels produce identical words:
-//  * msc_half_lockstep: one word per lane; a wave pays the maximum call count of its 64 words.
-//  * msc_half_queue: a wave owns a tile of 64*K words. Phase 1 computes (a0, a1, own ^ a2) of
-//    every word with coalesced loads into LDS. Phase 2 is a wave-local work queue: a lane whose
-//    word is decided takes the next undecided word (ballot + mbcnt rank, wave-uniform queue
-//    head in a scalar register), so every lane issues a useful Philox call on every iteration
-//    until the tile runs dry; the cost is the tile's mean, not its per-wave maximum. Phase 3
-//    writes the tile back with coalesced stores. Any lane may evaluate any (word, level) call
-//    because the stream is counter-based.
+// Cost model (measured with SQ counters on MI355X): Philox4x32-10 is 36 VALU ops, every op
+// including v_mad_u64_u32 at the full wave64 rate, so the kernel is VALU-issue bound and its
+// time is (lane-slots per word) x (ops per slot). Three things keep both factors low:
+//  * the level index lp is wave-uniform everywhere, so the threshold bits of a level are
+//    known to the whole wave: the host packs the 4-bit case code of every level pair into one
+//    64-bit argument and each call costs one scalar switch into straight-line v_bitop3 code
+//    (0-3 ops per 32-bit half and level) instead of a masked general update;
+//  * the loop is wave-uniform (no per-lane exit), so there is no exec-mask bookkeeping; a
+//    decided lane's updates are no-ops;
+//  * survivor compaction (mode 0): lockstep pays the slowest of 64 words; after `cap` levels
+//    only ~18% of words are undecided at T_c, so those are regrouped through LDS into full
+//    waves that start at the same level (still uniform) and run to completion.
 #include "common.hpp"
 
 struct MscArgs {
@@ -31,16 +35,21 @@ struct MscArgs {
     const u64* other;
     u32 rows, nw;
     u32 n_words;  // rows * nw (< 2^32, checked on the host)
-    u32 t, color, k0, k1, thr4, thr8;
+    u32 t, color, k0, k1;
     FastDiv div_nw;
 };
 
-struct MscWord {
-    u64 o, a0, a1, a2;
+// Lazy state of one word as 32-bit halves: a1 (lanes with exactly one anti-aligned
+// neighbour), und (undecided lanes, a subset of a0 | a1, so a0 = ~a1 on und) and pw (the word
+// being built: own ^ a2 ^ lanes accepted so far; accepted sets of successive levels are
+// disjoint, so they are XORed in).
+struct MscState {
+    u32 a1l, a1h, pwl, pwh, ul, uh;
 };
 
-// Neighbour classification of word w (global index), loads through the cache hierarchy.
-__device__ __forceinline__ MscWord msc_classify(const MscArgs& a, u32 w)
+// Neighbour classification of word w: five loads through the cache hierarchy and bit-sliced
+// counting of anti-aligned neighbours.
+__device__ __forceinline__ MscState msc_classify(const MscArgs& a, u32 w)
 {
     const u32 y = fast_div(w, a.div_nw);
     const u32 x = w - y * a.nw;
@@ -57,319 +66,234 @@ __device__ __forceinline__ MscWord msc_classify(const MscArgs& a, u32 w)
     const u64 dn = a.other[(u64)yd * a.nw + x];
     const u64 x1 = o ^ c, x2 = o ^ hz, x3 = o ^ up, x4 = o ^ dn;
     const u64 anyx = x1 | x2 | x3 | x4;
-    MscWord r;
-    r.o = o;
-    r.a2 = (x1 & x2) | (x3 & x4) | ((x1 ^ x2) & (x3 ^ x4));
-    r.a0 = ~anyx;
-    r.a1 = anyx & ~r.a2;
-    return r;
+    const u64 a2 = (x1 & x2) | (x3 & x4) | ((x1 ^ x2) & (x3 ^ x4));
+    const u64 a1 = anyx & ~a2;
+    const u64 und = ~a2;  // a0 | a1
+    const u64 pw = o ^ a2;
+    MscState s;
+    s.a1l = (u32)a1;
+    s.a1h = (u32)(a1 >> 32);
+    s.pwl = (u32)pw;
+    s.pwh = (u32)(pw >> 32);
+    s.ul = (u32)und;
+    s.uh = (u32)(und >> 32);
+    return s;
 }
 
-// Two lazy comparison levels from one Philox call (levels 2 lp and 2 lp + 1).
-__device__ __forceinline__ void msc_lazy_pair(const PhiloxOut& r, u32 lp, u32 thr4, u32 thr8,
-                                              u64 a0, u64 a1, u64& acc, u64& und)
+// ------------------------------------------------------------------ lazy comparison ----
+// Three-input boolean functions as single v_bitop3_b32 ops; the immediate is the function
+// evaluated on the operand patterns (0xF0, 0xCC, 0xAA).
+template <typename F>
+constexpr u32 b3imm(F f)
 {
-    const u64 lv0 = (u64)r.x | ((u64)r.y << 32);
-    const u64 lv1 = (u64)r.z | ((u64)r.w << 32);
-    const u32 b0 = 31u - 2u * lp;
-    const u64 t4a = ((thr4 >> b0) & 1u) ? ~0ull : 0ull;
-    const u64 t8a = ((thr8 >> b0) & 1u) ? ~0ull : 0ull;
-    const u64 tma = (a1 & t4a) | (a0 & t8a);
-    acc |= und & ~lv0 & tma;
-    und &= ~(lv0 ^ tma);
-    const u32 b1 = b0 - 1u;
-    const u64 t4b = ((thr4 >> b1) & 1u) ? ~0ull : 0ull;
-    const u64 t8b = ((thr8 >> b1) & 1u) ? ~0ull : 0ull;
-    const u64 tmb = (a1 & t4b) | (a0 & t8b);
-    acc |= und & ~lv1 & tmb;
-    und &= ~(lv1 ^ tmb);
+    return f(0xF0u, 0xCCu, 0xAAu) & 0xFFu;
+}
+constexpr u32 B3_A_ANDN_B = b3imm([](u32 x, u32 y, u32) { return x & ~y; });
+constexpr u32 B3_A_XOR_B_ANDN_C = b3imm([](u32 x, u32 y, u32 z) { return x ^ (y & ~z); });
+constexpr u32 B3_A_B_ANDN_C = b3imm([](u32 x, u32 y, u32 z) { return x & y & ~z; });
+constexpr u32 B3_A_XNOR_BC = b3imm([](u32 x, u32 y, u32 z) { return x & ~(y ^ z); });
+constexpr u32 B3_A_ANDN_B_ANDN_C = b3imm([](u32 x, u32 y, u32 z) { return x & ~y & ~z; });
+constexpr u32 B3_A_AND_B_XOR_C = b3imm([](u32 x, u32 y, u32 z) { return x & (y ^ z); });
+static_assert(B3_A_ANDN_B == 0x30u && B3_A_XOR_B_ANDN_C == 0xB4u && B3_A_B_ANDN_C == 0x40u &&
+                  B3_A_XNOR_BC == 0x90u && B3_A_ANDN_B_ANDN_C == 0x10u && B3_A_AND_B_XOR_C == 0x60u,
+              "bitop3 truth tables");
+
+#define bop3(a, b, c, imm) ((u32)__builtin_amdgcn_bitop3_b32((a), (b), (c), (imm)))
+
+// One level on a 32-bit half for threshold bits (B4 of thr4, B8 of thr8): a1 lanes compare
+// their uniform's bit with B4, a0 lanes with B8; a lane whose bit differs is decided (u bit 0
+// under a threshold bit 1 accepts, the other way rejects).
+template <u32 B4, u32 B8>
+__device__ __forceinline__ void msc_half_level(u32 lv, u32 a1, u32& pw, u32& und)
+{
+    if constexpr (B4 && B8) {  // every undecided lane sees a 1 bit
+        pw = bop3(pw, und, lv, B3_A_XOR_B_ANDN_C);
+        und &= lv;
+    } else if constexpr (B4) {  // a1 lanes see 1, a0 lanes 0
+        pw ^= bop3(und, a1, lv, B3_A_B_ANDN_C);
+        und = bop3(und, lv, a1, B3_A_XNOR_BC);
+    } else if constexpr (B8) {  // a0 lanes (und & ~a1) see 1, a1 lanes 0
+        pw ^= bop3(und, a1, lv, B3_A_ANDN_B_ANDN_C);
+        und = bop3(und, lv, a1, B3_A_AND_B_XOR_C);
+    } else {  // every undecided lane sees a 0 bit: a 1 in u rejects
+        und = bop3(und, lv, 0u, B3_A_ANDN_B);
+    }
 }
 
-// stats (optional, diagnostics only): [0] += Philox calls issued for real words,
-// [1] += 64 * per-wave iterations (the lane-slots the SIMD executes), [2] += words.
-template <bool LAZY>
-__global__ __launch_bounds__(256) void msc_half_lockstep(MscArgs a, unsigned long long* stats)
+// Both levels of one Philox call. CODE: bit 0 / 1 = thr4 / thr8 bit of level 2 lp, bits 2 / 3
+// the same for level 2 lp + 1.
+template <u32 CODE>
+__device__ __forceinline__ void msc_pair(const PhiloxOut& r, MscState& s)
 {
-    const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
-    const u32 w = blk * blockDim.x + threadIdx.x;
+    msc_half_level<CODE & 1u, (CODE >> 1) & 1u>(r.x, s.a1l, s.pwl, s.ul);
+    msc_half_level<CODE & 1u, (CODE >> 1) & 1u>(r.y, s.a1h, s.pwh, s.uh);
+    msc_half_level<(CODE >> 2) & 1u, (CODE >> 3) & 1u>(r.z, s.a1l, s.pwl, s.ul);
+    msc_half_level<(CODE >> 2) & 1u, (CODE >> 3) & 1u>(r.w, s.a1h, s.pwh, s.uh);
+}
+
+__device__ __forceinline__ void msc_pair_switch(u32 code, const PhiloxOut& r, MscState& s)
+{
+    switch (code) {  // code is wave-uniform: a scalar branch tree
+    case 0: msc_pair<0>(r, s); break;
+    case 1: msc_pair<1>(r, s); break;
+    case 2: msc_pair<2>(r, s); break;
+    case 3: msc_pair<3>(r, s); break;
+    case 4: msc_pair<4>(r, s); break;
+    case 5: msc_pair<5>(r, s); break;
+    case 6: msc_pair<6>(r, s); break;
+    case 7: msc_pair<7>(r, s); break;
+    case 8: msc_pair<8>(r, s); break;
+    case 9: msc_pair<9>(r, s); break;
+    case 10: msc_pair<10>(r, s); break;
+    case 11: msc_pair<11>(r, s); break;
+    case 12: msc_pair<12>(r, s); break;
+    case 13: msc_pair<13>(r, s); break;
+    case 14: msc_pair<14>(r, s); break;
+    default: msc_pair<15>(r, s); break;
+    }
+}
+
+// Nibble lp of the result = case code of levels 2 lp and 2 lp + 1 (host side, once per call).
+static inline u64 msc_level_codes(u32 thr4, u32 thr8)
+{
+    u64 codes = 0;
+    for (u32 lp = 0; lp < 16; ++lp) {
+        const u32 b0 = 31u - 2u * lp, b1 = b0 - 1u;
+        const u64 c = ((thr4 >> b0) & 1u) | (((thr8 >> b0) & 1u) << 1) |
+                      (((thr4 >> b1) & 1u) << 2) | (((thr8 >> b1) & 1u) << 3);
+        codes |= c << (4u * lp);
+    }
+    return codes;
+}
+
+// Wave-uniform lockstep loop over levels lp .. lp_end-1. LAZY stops once every lane of the
+// wave is decided; the eager form always runs to lp_end. Returns the level reached.
+template <bool LAZY, bool STATS>
+__device__ __forceinline__ u32 msc_run(const MscArgs& a, u64 codes, u32 w, u32 lp, u32 lp_end,
+                                       MscState& s, u32& calls, u32& slots)
+{
+    while (lp < lp_end && (!LAZY || __ballot((s.ul | s.uh) != 0u) != 0)) {
+        if (STATS) {
+            calls += (!LAZY || (s.ul | s.uh) != 0u) ? 1u : 0u;
+            ++slots;
+        }
+        const PhiloxOut r = philox4x32_10(w, a.t, a.color, 1u + lp, a.k0, a.k1);
+        msc_pair_switch((u32)(codes >> (4u * lp)) & 15u, r, s);
+        ++lp;
+    }
+    return lp;
+}
+
+__device__ __forceinline__ void msc_stats(unsigned long long* stats, u32 calls, u32 slots,
+                                          u32 words)
+{
+    u64 tot = calls, nwd = words;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        tot += __shfl_xor(tot, off, 64);
+        nwd += __shfl_xor(nwd, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && nwd) {
+        atomicAdd(stats + 0, (unsigned long long)tot);
+        atomicAdd(stats + 1, (unsigned long long)slots * 64ull);
+        atomicAdd(stats + 2, (unsigned long long)nwd);
+    }
+}
+
+// ---------------------------------------------------------------------------- kernels ----
+// stats (optional, diagnostics only): [0] += Philox calls the words needed, [1] += lane-slots
+// the SIMDs executed (64 x wave iterations), [2] += words.
+
+// Lockstep: one word per lane, the wave pays the maximum call count of its 64 words.
+// LAZY = false is the eager ablation (all 16 calls, src/pyising/backends/cuda/ising.cu:135).
+template <bool LAZY, bool STATS>
+__global__ __launch_bounds__(256) void msc_half_lockstep(MscArgs a, u64 codes,
+                                                        unsigned long long* stats)
+{
+    const u32 w = xcd_band_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool valid = w < a.n_words;
-    u32 calls = 0;
-    if (valid) {
-        const MscWord m = msc_classify(a, w);
-        u64 acc = 0, und = m.a0 | m.a1;
-        for (u32 lp = 0; lp < 16u && (!LAZY || und); ++lp) {
-            const PhiloxOut r = philox4x32_10(w, a.t, a.color, 1u + lp, a.k0, a.k1);
-            ++calls;
-            msc_lazy_pair(r, lp, a.thr4, a.thr8, m.a0, m.a1, acc, und);
-        }
-        a.own[w] = m.o ^ (m.a2 | acc);
-    }
-    if (stats) {
-        u32 wmax = calls;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, (u32)__shfl_xor((int)wmax, off, 64));
-        u64 tot = calls;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, 64);
-        const u64 nval = __popcll(__ballot(valid));
-        if ((threadIdx.x & 63) == 0 && nval) {
-            atomicAdd(stats + 0, (unsigned long long)tot);
-            atomicAdd(stats + 1, (unsigned long long)wmax * 64ull);
-            atomicAdd(stats + 2, (unsigned long long)nval);
-        }
-    }
+    MscState s = {};
+    if (valid) s = msc_classify(a, w);
+    u32 calls = 0, slots = 0;
+    msc_run<LAZY, STATS>(a, codes, w, 0u, 16u, s, calls, slots);
+    if (valid) a.own[w] = (u64)s.pwl | ((u64)s.pwh << 32);
+    if (STATS) msc_stats(stats, calls, slots, valid ? 1u : 0u);
 }
 
-__device__ __forceinline__ u32 lane_rank(u64 mask)  // set bits of mask below this lane
+// Lockstep + survivor compaction (the default). A 256-thread workgroup owns W = 256 * R
+// consecutive words.
+// Phase A: R rounds of one word per lane, lockstep levels 0 .. cap-1 (nearly every word needs
+// them at T_c); decided words are stored, the others' state (a1, und, pw) parked in LDS and
+// their slot appended to a queue (one LDS atomic per wave). One barrier.
+// Phase B: waves take 64 queued survivors at a time (LDS batch counter) and run them lockstep
+// from level cap to completion; all start at the same level, so the level index stays
+// uniform, and the wave pays only the maximum of the survivors' remaining calls.
+// R = 2 uses 13 KB of LDS: 8 workgroups (32 waves) per CU.
+template <int R, bool STATS>
+__global__ __launch_bounds__(256) void msc_half_compact(MscArgs a, u64 codes, u32 cap,
+                                                       unsigned long long* stats)
 {
-    return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
-}
-
-template <int K>
-__global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u32 n_tiles,
-                                                     unsigned long long* stats)
-{
-    constexpr u32 T = 64u * K;  // words per wave tile
-    __shared__ u64 lds[4][3][T];
-    const u32 wave = threadIdx.x >> 6;
-    const u32 lane = threadIdx.x & 63u;
-    u64* L0 = lds[wave][0];
-    u64* L1 = lds[wave][1];
-    u64* LP = lds[wave][2];
-    const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
-    const u32 tile = blk * 4u + wave;
-    const bool tile_ok = tile < n_tiles;
-    const u32 base = tile * T;
-
-    // phase 1: classify every word of the tile (coalesced: word base + i * 64 + lane)
-    if (tile_ok) {
+    constexpr u32 W = 256u * R;
+    __shared__ u64 s_a1[W], s_und[W], s_pw[W];
+    __shared__ uint16_t s_q[W];
+    __shared__ u32 s_qn, s_next;
+    const u32 tid = threadIdx.x;
+    const u32 lane = tid & 63u;
+    const u32 base = xcd_band_block(blockIdx.x, gridDim.x) * W;
+    if (tid == 0) {
+        s_qn = 0;
+        s_next = 0;
+    }
+    __syncthreads();
+    u32 calls = 0, slots = 0, words = 0;
 #pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const u32 s = (u32)i * 64u + lane;
-            const u32 w = base + s;
-            if (w < a.n_words) {
-                const MscWord m = msc_classify(a, w);
-                L0[s] = m.a0;
-                L1[s] = m.a1;
-                LP[s] = m.o ^ m.a2;
-            } else {
-                L0[s] = 0;
-                L1[s] = 0;
-            }
+    for (int rr = 0; rr < R; ++rr) {
+        const u32 slot = (u32)rr * 256u + tid;
+        const u32 w = base + slot;
+        const bool valid = w < a.n_words;
+        MscState s = {};
+        if (valid) {
+            s = msc_classify(a, w);
+            ++words;
+        }
+        msc_run<true, STATS>(a, codes, w, 0u, cap, s, calls, slots);
+        const bool park = valid && (s.ul | s.uh) != 0u;
+        if (valid && !park) a.own[w] = (u64)s.pwl | ((u64)s.pwh << 32);
+        const u64 pm = __ballot(park);
+        u32 qb = 0;
+        if (lane == 0 && pm) qb = atomicAdd(&s_qn, (u32)__popcll(pm));
+        qb = __shfl(qb, 0, 64);
+        if (park) {
+            s_a1[slot] = (u64)s.a1l | ((u64)s.a1h << 32);
+            s_und[slot] = (u64)s.ul | ((u64)s.uh << 32);
+            s_pw[slot] = (u64)s.pwl | ((u64)s.pwh << 32);
+            s_q[qb + lane_rank(pm)] = (uint16_t)slot;
         }
     }
     __syncthreads();
-
-    // phase 2: wave-local queue over the tile's undecided words
-    u64 iters = 0, calls = 0;
-    if (tile_ok) {
-        u32 head = 0;  // wave-uniform queue head
-        u32 cur = 0, lp = 0;
-        u64 a0 = 0, a1 = 0, acc = 0, und = 0;
-        bool live = true;
-        const u32 wbase = base;  // Philox word counter
-        while (true) {
-            bool need = live && (und == 0);
-            u64 m = __ballot(need);
-            while (m) {
-                const u32 pos = head + lane_rank(m);
-                head += (u32)__popcll(m);
-                if (need) {
-                    if (pos < T) {
-                        cur = pos;
-                        a0 = L0[pos];
-                        a1 = L1[pos];
-                        und = a0 | a1;
-                        acc = 0;
-                        lp = 0;
-                        need = (und == 0);  // word decided without random numbers
-                    } else {
-                        live = false;
-                        need = false;
-                    }
-                }
-                m = __ballot(need);
-            }
-            if (__ballot(und != 0) == 0) break;
-            ++iters;
-            if (und) {
-                const PhiloxOut r = philox4x32_10(wbase + cur, a.t, a.color, 1u + lp, a.k0, a.k1);
-                ++calls;
-                msc_lazy_pair(r, lp, a.thr4, a.thr8, a0, a1, acc, und);
-                ++lp;
-                if (lp == 16u) und = 0;  // all 32 bits compared: u == thr, not accepted
-                if (und == 0) LP[cur] ^= acc;
-            }
-        }
-    }
-    __syncthreads();
-
-    // phase 3: coalesced write-back
-    if (tile_ok) {
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const u32 s = (u32)i * 64u + lane;
-            const u32 w = base + s;
-            if (w < a.n_words) a.own[w] = LP[s];
-        }
-    }
-    if (stats && tile_ok) {
-        u64 tot = calls;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, 64);
-        if (lane == 0) {
-            const u32 nw = (a.n_words - base < T) ? (a.n_words - base) : T;
-            atomicAdd(stats + 0, (unsigned long long)tot);
-            atomicAdd(stats + 1, (unsigned long long)iters * 64ull);
-            atomicAdd(stats + 2, (unsigned long long)nw);
-        }
-    }
-}
-
-// Two lazy levels with per-lane level index lp (the work-queue kernels): the threshold bit of
-// level b is a sign-extended one-bit field, 0 or ~0, of the scalar threshold.
-__device__ __forceinline__ void msc_lazy_pair_lane(const PhiloxOut& r, u32 lp, u32 thr4, u32 thr8,
-                                                   u64 a0, u64 a1, u64& acc, u64& und)
-{
-    const u32 b0 = 31u - 2u * lp;
-    const u32 lvs[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-    for (int lev = 0; lev < 2; ++lev) {
-        const u32 b = b0 - (u32)lev;
-        const u32 t4 = (u32)__builtin_amdgcn_sbfe((int)thr4, b, 1);
-        const u32 t8 = (u32)__builtin_amdgcn_sbfe((int)thr8, b, 1);
-        u32 ac[2] = {(u32)acc, (u32)(acc >> 32)};
-        u32 un[2] = {(u32)und, (u32)(und >> 32)};
-        const u32 x0[2] = {(u32)a0, (u32)(a0 >> 32)};
-        const u32 x1[2] = {(u32)a1, (u32)(a1 >> 32)};
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const u32 lv = lvs[2 * lev + h];
-            const u32 tm = (x1[h] & t4) | (x0[h] & t8);
-            ac[h] |= un[h] & ~lv & tm;
-            un[h] &= ~(lv ^ tm);
-        }
-        acc = (u64)ac[0] | ((u64)ac[1] << 32);
-        und = (u64)un[0] | ((u64)un[1] << 32);
-    }
-}
-
-// Streaming work queue. Each wave owns `groups` consecutive groups of 64 words. A 128-slot
-// ring in LDS holds classified, unclaimed words (a0, a1, own ^ a2); slot counter c maps to
-// word wbase + c, so no index is stored. A lane claims the next slot when its word is
-// decided (ballot + mbcnt rank), copies the slot into registers (the slot is free at once),
-// and stores the finished word straight to `own`. The ring is topped up with the next group
-// (coalesced loads by all 64 lanes) whenever it holds <= 64 words, so lanes only idle in the
-// final drain of the wave's range. 3 KB of LDS per wave keeps 8 waves per SIMD resident.
-__global__ __launch_bounds__(256) void msc_half_stream(MscArgs a, u32 n_groups, u32 groups,
-                                                      unsigned long long* stats)
-{
-    __shared__ u64 ring[4][3][128];
-    const u32 wave = threadIdx.x >> 6;
-    const u32 lane = threadIdx.x & 63u;
-    u64* R0 = ring[wave][0];
-    u64* R1 = ring[wave][1];
-    u64* RP = ring[wave][2];
-    const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
-    const u32 wid = blk * 4u + wave;
-    const u32 g_begin = wid * groups;
-    if (g_begin >= n_groups) return;  // whole wave exits; no block barriers below
-    const u32 g_end = min(g_begin + groups, n_groups);
-    const u32 wbase = g_begin * 64u;
-
-    u32 gnext = g_begin;
-    u32 head = 0, tail = 0;  // wave-uniform ring counters
-    bool has = false;
-    u32 word = 0, lp = 0;
-    u64 a0 = 0, a1 = 0, pw = 0, acc = 0, und = 0;
-    u64 iters = 0, calls = 0;
-
+    const u32 n = s_qn;
     while (true) {
-        // claim words for idle lanes, topping the ring up when it runs dry
-        while (true) {
-            const bool need = !has;
-            const u64 m = __ballot(need);
-            if (m == 0) break;
-            if (tail == head) {
-                if (gnext >= g_end) break;
-                const u32 w = gnext * 64u + lane;
-                const u32 s = tail & 127u;
-                if (w < a.n_words) {
-                    const MscWord cw = msc_classify(a, w);
-                    R0[s] = cw.a0;
-                    R1[s] = cw.a1;
-                    RP[s] = cw.o ^ cw.a2;
-                } else {
-                    R0[s] = 0;
-                    R1[s] = 0;
-                }
-                __builtin_amdgcn_wave_barrier();
-                tail += 64u;
-                ++gnext;
-            }
-            const u32 avail = tail - head;
-            const u32 rank = lane_rank(m);
-            if (need && rank < avail) {
-                const u32 c = head + rank;
-                const u32 s = c & 127u;
-                word = wbase + c;
-                a0 = R0[s];
-                a1 = R1[s];
-                pw = RP[s];
-                und = a0 | a1;
-                acc = 0;
-                lp = 0;
-                has = true;
-                if (und == 0) {  // decided without random numbers
-                    if (word < a.n_words) a.own[word] = pw;
-                    has = false;
-                }
-            }
-            const u32 taken = (u32)__popcll(m);
-            head += taken < avail ? taken : avail;
+        u32 b = 0;
+        if (lane == 0) b = atomicAdd(&s_next, 64u);
+        b = __shfl(b, 0, 64);
+        if (b >= n) break;  // wave-uniform
+        const u32 i = b + lane;
+        MscState s = {};
+        u32 slot = 0;
+        if (i < n) {
+            slot = s_q[i];
+            const u64 a1 = s_a1[slot], und = s_und[slot], pw = s_pw[slot];
+            s.a1l = (u32)a1;
+            s.a1h = (u32)(a1 >> 32);
+            s.ul = (u32)und;
+            s.uh = (u32)(und >> 32);
+            s.pwl = (u32)pw;
+            s.pwh = (u32)(pw >> 32);
         }
-        // keep one group queued ahead
-        if (tail - head <= 64u && gnext < g_end) {
-            const u32 w = gnext * 64u + lane;
-            const u32 s = tail & 127u;
-            if (w < a.n_words) {
-                const MscWord cw = msc_classify(a, w);
-                R0[s] = cw.a0;
-                R1[s] = cw.a1;
-                RP[s] = cw.o ^ cw.a2;
-            } else {
-                R0[s] = 0;
-                R1[s] = 0;
-            }
-            __builtin_amdgcn_wave_barrier();
-            tail += 64u;
-            ++gnext;
-        }
-        if (__ballot(has) == 0) break;
-        ++iters;
-        if (has) {
-            const PhiloxOut r = philox4x32_10(word, a.t, a.color, 1u + lp, a.k0, a.k1);
-            ++calls;
-            msc_lazy_pair_lane(r, lp, a.thr4, a.thr8, a0, a1, acc, und);
-            ++lp;
-            if (lp == 16u) und = 0;  // all 32 bits compared: u == thr, not accepted
-            if (und == 0) {
-                a.own[word] = pw ^ acc;
-                has = false;
-            }
-        }
+        msc_run<true, STATS>(a, codes, base + slot, cap, 16u, s, calls, slots);
+        if (i < n) a.own[base + slot] = (u64)s.pwl | ((u64)s.pwh << 32);
     }
-    if (stats) {
-        u64 tot = calls;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, 64);
-        if (lane == 0) {
-            const u32 wend = min(g_end * 64u, a.n_words);
-            atomicAdd(stats + 0, (unsigned long long)tot);
-            atomicAdd(stats + 1, (unsigned long long)iters * 64ull);
-            atomicAdd(stats + 2, (unsigned long long)(wend - wbase));
-        }
-    }
+    if (STATS) msc_stats(stats, calls, slots, words);
 }
 
 // ------------------------------------------------------------------------ observables ----
@@ -409,6 +333,7 @@ __global__ __launch_bounds__(256) void msc_obs_kernel(const u64* __restrict__ w0
     }
 }
 
+// out = (M, E); also clears the accumulator for the next call.
 __global__ void msc_obs_finalize(unsigned long long* acc, long long* out, long long n_spins)
 {
     const long long ones = (long long)acc[0], anti = (long long)acc[1];
@@ -424,11 +349,11 @@ __global__ void msc_zero2(unsigned long long* p)
 }
 
 // ---------------------------------------------------------------------- host launchers ----
-// Kernel modes: 0 = tile work queue (tile of 64*K words per wave, K in {2,4,8,16}),
-// 1 = lockstep (one word per lane), 2 = eager lockstep (all 16 calls, ablation),
-// 3 = streaming work queue (K = groups of 64 words per wave, 1..64).
+// Kernel modes: 0 = lockstep + survivor compaction (k = 10 * cap + R, cap in 1..15,
+// R in {1, 2, 4}; default cap 4, R 2), 1 = lockstep, 2 = eager lockstep (all 16 calls per word,
+// the msc-eager32 ablation; k unused for 1 and 2).
 static int g_msc_mode = 0;
-static int g_msc_k = 8;
+static int g_msc_k = 42;
 
 static int msc_check(const void* own, const void* other, long long rows, int nw)
 {
@@ -442,13 +367,25 @@ static int msc_check(const void* own, const void* other, long long rows, int nw)
 static int msc_mode_ok(int mode, int k)
 {
     if (mode < 0) return ISING_OK;  // process default
-    if (mode > 3)
-        return ising_fail(ISING_EINVAL, "msc: mode must be 0 (tile queue), 1 (lockstep), 2 (eager) or 3 (stream)");
-    if (mode == 0 && k != 2 && k != 4 && k != 8 && k != 16)
-        return ising_fail(ISING_EINVAL, "msc: tile K must be 2, 4, 8 or 16");
-    if (mode == 3 && (k < 1 || k > 64))
-        return ising_fail(ISING_EINVAL, "msc: stream groups per wave must be in 1..64");
+    if (mode > 2)
+        return ising_fail(ISING_EINVAL, "msc: mode must be 0 (compaction), 1 (lockstep) or 2 (eager)");
+    if (mode == 0) {
+        const int cap = k / 10, rr = k % 10;
+        if (cap < 1 || cap > 15 || (rr != 1 && rr != 2 && rr != 4))
+            return ising_fail(ISING_EINVAL,
+                              "msc: compaction k = 10 * cap + R with cap in 1..15, R in {1,2,4}");
+    }
     return ISING_OK;
+}
+
+template <int R>
+static void launch_compact(dim3 grid, hipStream_t st, const MscArgs& a, u64 codes, u32 cap,
+                           unsigned long long* stats)
+{
+    if (stats)
+        hipLaunchKernelGGL((msc_half_compact<R, true>), grid, dim3(256), 0, st, a, codes, cap, stats);
+    else
+        hipLaunchKernelGGL((msc_half_compact<R, false>), grid, dim3(256), 0, st, a, codes, cap, stats);
 }
 
 static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32 t, u32 color,
@@ -474,31 +411,21 @@ static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32
     a.color = color;
     a.k0 = k0;
     a.k1 = k1;
-    a.thr4 = thr4;
-    a.thr8 = thr8;
-    if (mode == 3) {
-        const u32 n_groups = (a.n_words + 63u) / 64u;
-        const u32 waves = (n_groups + (u32)k - 1u) / (u32)k;
-        const u32 blocks = (waves + 3u) / 4u;
-        hipLaunchKernelGGL(msc_half_stream, dim3(blocks), dim3(256), 0, st, a, n_groups, (u32)k, stats);
-    } else if (mode != 0) {
-        const u64 blocks = (a.n_words + 255) / 256;
-        if (blocks > 0x7FFFFFFFull) return ising_fail(ISING_ERANGE, "msc: lattice too large");
-        if (mode == 1)
-            hipLaunchKernelGGL(msc_half_lockstep<true>, dim3((unsigned)blocks), dim3(256), 0, st, a, stats);
-        else
-            hipLaunchKernelGGL(msc_half_lockstep<false>, dim3((unsigned)blocks), dim3(256), 0, st, a, stats);
+    const u64 codes = msc_level_codes(thr4, thr8);
+    if (mode == 0) {
+        const u32 cap = (u32)(k / 10), rr = (u32)(k % 10);
+        const dim3 grid((a.n_words + 256u * rr - 1u) / (256u * rr));
+        if (rr == 1) launch_compact<1>(grid, st, a, codes, cap, stats);
+        else if (rr == 2) launch_compact<2>(grid, st, a, codes, cap, stats);
+        else launch_compact<4>(grid, st, a, codes, cap, stats);
     } else {
-        const u64 T = 64ull * (u64)k;
-        const u64 n_tiles = (a.n_words + T - 1) / T;
-        const u64 blocks = (n_tiles + 3) / 4;
-        if (n_tiles > 0xFFFFFFFFull) return ising_fail(ISING_ERANGE, "msc: lattice too large");
-        const dim3 grid((unsigned)blocks);
-        switch (k) {
-        case 2: hipLaunchKernelGGL(msc_half_queue<2>, grid, dim3(256), 0, st, a, (u32)n_tiles, stats); break;
-        case 4: hipLaunchKernelGGL(msc_half_queue<4>, grid, dim3(256), 0, st, a, (u32)n_tiles, stats); break;
-        case 16: hipLaunchKernelGGL(msc_half_queue<16>, grid, dim3(256), 0, st, a, (u32)n_tiles, stats); break;
-        default: hipLaunchKernelGGL(msc_half_queue<8>, grid, dim3(256), 0, st, a, (u32)n_tiles, stats); break;
+        const dim3 grid((a.n_words + 255u) / 256u);
+        if (mode == 1) {
+            if (stats) hipLaunchKernelGGL((msc_half_lockstep<true, true>), grid, dim3(256), 0, st, a, codes, stats);
+            else hipLaunchKernelGGL((msc_half_lockstep<true, false>), grid, dim3(256), 0, st, a, codes, stats);
+        } else {
+            if (stats) hipLaunchKernelGGL((msc_half_lockstep<false, true>), grid, dim3(256), 0, st, a, codes, stats);
+            else hipLaunchKernelGGL((msc_half_lockstep<false, false>), grid, dim3(256), 0, st, a, codes, stats);
         }
     }
     ISING_LAUNCHED("msc_half");
@@ -528,7 +455,7 @@ int ising_msc_config(int mode, int k)
     int rc = msc_mode_ok(mode, k);
     if (rc) return rc;
     g_msc_mode = mode;
-    g_msc_k = (mode == 0 || mode == 3) ? k : g_msc_k;
+    if (mode == 0) g_msc_k = k;
     return ISING_OK;
 }
 

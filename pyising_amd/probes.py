@@ -119,10 +119,16 @@ def main(argv=None) -> int:
     i.add_argument("--dim", type=int, default=2)
     i.add_argument("--sweeps", type=int, default=10)
     i.add_argument("--algorithm", default="metropolis")
+    ph = sub.add_parser("philox")
+    ph.add_argument("--calls", type=int, default=512)
+    ph.add_argument("--reps", type=int, default=3)
     sub.add_parser("ceilings")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         out = _run_copy(a.mib, a.reps)
+    elif a.cmd == "philox":
+        rate = philox_rate(calls=a.calls, repeats=a.reps)
+        out = {"probe": "philox", "calls_per_thread": a.calls, "philox_calls_per_ns": rate}
     elif a.cmd == "msc":
         out = _run_msc(a.L, a.sweeps, a.mode, a.k)
     elif a.cmd == "int8":
