@@ -171,6 +171,4 @@ def main() -> int:
 
 
 if __name__ == "__main__":
-    code = main()
-    sys.stdout.flush()
-    os._exit(code)
+    sys.exit(main())

@@ -113,7 +113,7 @@ def main(argv=None) -> int:
     m.add_argument("--L", type=int, default=16384)
     m.add_argument("--sweeps", type=int, default=10)
     m.add_argument("--mode", type=int, default=0)
-    m.add_argument("--k", type=int, default=8)
+    m.add_argument("--k", type=int, default=42)
     i = sub.add_parser("int8")
     i.add_argument("--L", type=int, default=8192)
     i.add_argument("--dim", type=int, default=2)

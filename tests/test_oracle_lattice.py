@@ -116,7 +116,7 @@ def test_msc_oracle_matches_eager_definition(shape):
 
 def test_msc_at_beta_zero_matches_int8_rule():
     """At beta = 0 every threshold is UINT32_MAX: both streams flip every spin whose uniform is
-    not 0xFFFFFFFF (src/pyising/.. tests/test_lattice.py:91-99 of the reference)."""
+    not 0xFFFFFFFF (tests/test_lattice.py:91-99 of the reference)."""
     lat = O.random_lattice((16, 128), 4)
     a = O.MscOracle((16, 128), 0.0, seed=5, init=lat)
     b = O.Int8Oracle((16, 128), 0.0, seed=5, init=lat)
