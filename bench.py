@@ -64,6 +64,49 @@ def cpu_baseline(seconds: float = 10.0) -> dict:
                       f"(os.cpu_count()={os.cpu_count()})"}
 
 
+def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
+    """Binder temperature scan: 96 int8 lattices (L = 64..2048) and 32 msc lattices
+    (L = 1024, 4096), 16 temperatures in [2.20, 2.34] each, seeds base + index, hot starts.
+    Aggregate flips/ns of the batched scan (all lattices per launch, hipGraph replays) versus
+    the plain per-lattice loop (``sim.sweep(n)`` lattice after lattice)."""
+    from pyising_amd import _lib
+    from pyising_amd.batch import BatchScan, temperature_scan_lattices
+
+    sims = temperature_scan_lattices()
+    n_spins = sum(s.n_spins for s in sims)
+    scan = BatchScan(sims)
+    per_replay = 50
+    graph = scan.capture(per_replay)
+    e0, e1 = _lib.Event(), _lib.Event()
+    if graph:
+        scan.replay(1)
+    else:
+        scan.sweep(per_replay)
+    scan.synchronize()
+    reps = max(1, batch_sweeps // per_replay)
+    e0.record(scan.stream.ptr)
+    if graph:
+        scan.replay(reps)
+    else:
+        scan.sweep(reps * per_replay)
+    e1.record(scan.stream.ptr)
+    batch_rate = reps * per_replay * n_spins / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+    mom = scan.moments()
+    scan.close()
+    for s in sims:
+        s.sweep(1)
+    _lib.device_sync()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.sweep(plain_sweeps)
+    _lib.device_sync()
+    plain_rate = plain_sweeps * n_spins / (time.perf_counter() - t0) / 1e9
+    return {"lattices": len(sims), "spins": int(n_spins), "graph": bool(graph),
+            "batched_flips_per_ns": round(batch_rate, 1), "plain_flips_per_ns": round(plain_rate, 1),
+            "speedup": round(batch_rate / plain_rate, 2),
+            "binder_L4096_Tmid": round(mom[-9].binder, 4)}
+
+
 def traffic_from_profile() -> int | None:
     """HBM bytes per msc half-sweep launch from the committed rocprofv3 PMC summary."""
     try:
@@ -80,6 +123,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ceilings", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path "
@@ -164,6 +208,8 @@ def main() -> int:
             "rng_ceiling_flips_per_ns": round(ph * 64 / stats["calls_per_word"], 1),
             "memory_ceiling_flips_per_ns": round(bw / BYTES_PER_UPDATE, 1),
         }
+    if not args.no_batch:
+        result["batched_scan"] = batch_vs_plain()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)

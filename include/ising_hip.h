@@ -134,6 +134,46 @@ int ising_msc_sweeps_ex(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw, 
 int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, int nw,
                   unsigned long long* d_scratch2, long long* d_out2, void* stream);
 
+/* ---------------------------------------------------------- batched scans ---- */
+/* No reference counterpart: the reference sweeps one lattice per launch sequence
+ * (src/pyising/backends/base.py:85-90). A batch is a list of lattices (int8 2D/3D either
+ * algorithm, or msc 2D), each with its own shape, thresholds, seed and sweep counter, whose
+ * device colour arrays stay owned by the caller. One persistent cooperative kernel sweeps
+ * them all (grid barrier between the two half sweeps); every lattice's result is
+ * bit-identical to sweeping it alone. After every sweep M = sum of spins is accumulated per
+ * lattice: sum M, sum |M|, sum M^2 (int64) and sum M^4 (128-bit) for Binder cumulants
+ * (src/pyising/observables.py:39-42). */
+typedef struct {
+    int kind;            /* 0 = int8 compressed checkerboard, 1 = msc packed */
+    int ndim;            /* 2 or 3 (msc: 2) */
+    int heatbath;        /* int8 only: 0 Metropolis, 1 heat bath */
+    int lz, ly, lx;      /* shape; lz = 1 in 2D */
+    unsigned long long seed;
+    uint32_t table[7];   /* thresholds (2*ndim+1 used; msc uses table[3], table[4]) */
+    void* c0;            /* device colour arrays (int8 (rows, lx/2) or uint64 (rows, lx/128)) */
+    void* c1;
+    uint32_t t0;         /* sweep counter at batch start */
+} IsingBatchLattice;
+/* blocks_per_cu <= 0: occupancy maximum; msc_mode 0 = compaction tiles (512 words),
+ * 1 = lockstep tiles (256 words); msc_cap = lockstep levels before compaction (1..15). */
+int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, int msc_mode,
+                       int msc_cap, void** handle);
+/* n_sweeps sweeps of every lattice in one cooperative launch (asynchronous). */
+int ising_batch_sweeps(void* handle, long long n_sweeps, void* stream);
+/* Capture one launch of sweeps_per_replay sweeps into a hipGraph on `stream` (a created
+ * stream); ISING_EHIP if the runtime cannot capture it. Replays continue the sweep counter. */
+int ising_batch_graph(void* handle, long long sweeps_per_replay, void* stream);
+int ising_batch_replay(void* handle, int replays, void* stream);
+/* Synchronous read of n x 6 int64 accumulators (sum M, sum |M|, sum M^2, sum M^4 lo, hi,
+ * samples) and the number of sweeps done. */
+int ising_batch_stats(void* handle, long long* out, long long* sweeps_done, void* stream);
+int ising_batch_reset_stats(void* handle, void* stream);
+int ising_batch_info(void* handle, int* grid, int* total_tiles, int* max_blocks_per_cu);
+/* Form of a sweep call: 0 (default) = one launch per half sweep + one accumulation launch per
+ * sweep; 1 = one persistent cooperative launch with a grid barrier between half sweeps. */
+int ising_batch_set_persistent(void* handle, int persistent);
+int ising_batch_destroy(void* handle);
+
 /* ------------------------------------------------------------------ probes ---- */
 /* Philox4x32-10 of n (counter[4], key[2]) pairs, host arrays in and out: the known-answer
  * path of philox4x32_10 (ising.cu:10-21; tests/test_philox.py:7-18). Synchronous. */

@@ -1,0 +1,532 @@
+// Batched temperature scans: one persistent kernel sweeps many lattices of uneven size,
+// dimension, variant and temperature in a single launch (no reference counterpart; the
+// reference runs one lattice per launch sequence, src/pyising/backends/base.py:85-90).
+//
+// Work decomposition. Every lattice is cut into tiles of one workgroup's work for one half
+// sweep: int8 = 256 threads x V sites, msc = 512 words (lockstep + survivor compaction) or 256
+// words (lockstep). A tile map lists (tile -> lattice) for all lattices; the grid strides over
+// it. Colour c + 1 of a lattice may only start when every tile of colour c of that lattice is
+// done: the kernel is launched cooperatively (all workgroups co-resident, checked by the
+// runtime) and a grid barrier separates the two half sweeps of every sweep, which orders all
+// lattices at once. Each tile is bit-identical to the single-lattice kernels (same device
+// functions, same counters: t = t0 + sweep), so a batched lattice equals the same lattice swept
+// alone.
+//
+// Two forms give identical results: a persistent cooperative launch per call (grid barrier
+// between half sweeps), and (default) one launch per half sweep over all lattices plus one
+// accumulation launch per sweep. On MI355X the kernel boundary is the cheaper barrier: a grid
+// barrier is a contended cross-XCD atomic on one address per workgroup.
+//
+// Binder accumulation. While writing a colour, each tile stores its number of +1 spins in its
+// own slot (slots double-buffered by sweep parity; no atomics, so no contention on a
+// per-lattice counter). After a sweep, one wave per lattice sums its slots, forms
+// M = 2 * plus - N and accumulates sum M, sum |M|, sum M^2 (int64: M^2 <= 2^48 for L <= 4096,
+// so 10^4 sweeps stay below 2^62) and sum M^4 as a 128-bit (hi, lo) pair. The number of sweeps
+// done lives in device memory, so a captured graph can be replayed and every replay continues
+// the counter stream.
+#include <hip/hip_cooperative_groups.h>
+
+#include <vector>
+
+#include "int8_core.hpp"
+#include "msc_core.hpp"
+
+namespace cg = cooperative_groups;
+
+enum { BATCH_INT8 = 0, BATCH_MSC = 1 };
+
+struct BatchDesc {
+    void* c0;
+    void* c1;
+    u32 kind, ndim, hb, vec;  // vec: int8 sites per thread (16, 8 or 4)
+    Int8Geom g;
+    Int8Table tb;
+    u32 rows, nw, n_words;
+    FastDiv div_nw;
+    u64 codes;
+    u32 k0, k1, t0;
+    u32 tile_begin, n_tiles;
+    long long n_spins;
+};
+
+struct BatchAcc {
+    long long sum_m;
+    unsigned long long sum_abs_m, sum_m2, sum_m4_lo, sum_m4_hi, samples;
+};
+
+struct BatchKernelArgs {
+    const BatchDesc* descs;
+    const u32* tile_lat;
+    u32* plus;  // [2 sweep parity][2 colour][total_tiles]: +1 spins written by each tile
+    BatchAcc* acc;
+    unsigned long long* sweeps_done;
+    u32 n_lat, total_tiles, n_sweeps, msc_mode, msc_cap;
+};
+
+// Whole-workgroup sum of v stored to *dst by thread 0 (no atomics: each tile owns its slot, so
+// there is no contention on a per-lattice counter).
+__device__ __forceinline__ void block_store_sum(u32* dst, u32 v)
+{
+    __shared__ u32 red[4];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) *dst = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+}
+
+// One wave: M of lattice `lat` from its tiles' counts of parity `par`, folded into the
+// accumulators (sum M, sum |M|, sum M^2, 128-bit sum M^4, samples).
+__device__ __forceinline__ void batch_accumulate_lattice(const BatchKernelArgs& ka, u32 lat, u32 par)
+{
+    const BatchDesc& d = ka.descs[lat];
+    const u32* p0 = ka.plus + ((size_t)par * 2 + 0) * ka.total_tiles + d.tile_begin;
+    const u32* p1 = ka.plus + ((size_t)par * 2 + 1) * ka.total_tiles + d.tile_begin;
+    u64 plus = 0;
+    for (u32 i = threadIdx.x & 63u; i < d.n_tiles; i += 64u) plus += (u64)p0[i] + p1[i];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) plus += __shfl_xor(plus, off, 64);
+    if ((threadIdx.x & 63) == 0) {
+        const long long m = 2 * (long long)plus - d.n_spins;
+        BatchAcc& A = ka.acc[lat];
+        const u64 m2 = (u64)(m * m);
+        const u64 lo = m2 * m2, hi = __umul64hi(m2, m2);
+        A.sum_m += m;
+        A.sum_abs_m += (u64)(m < 0 ? -m : m);
+        A.sum_m2 += m2;
+        const u64 nlo = A.sum_m4_lo + lo;
+        A.sum_m4_hi += hi + (nlo < lo ? 1u : 0u);
+        A.sum_m4_lo = nlo;
+        A.samples += 1;
+    }
+}
+
+template <int NDIM, int HB, int V>
+__device__ __forceinline__ u32 batch_int8_tile(const BatchDesc& d, u32 local, u32 t, u32 color)
+{
+    const u32 ci = local * 256u + threadIdx.x;
+    if (ci >= d.g.n_chunks) return 0u;
+    int8_t* own = (int8_t*)(color ? d.c1 : d.c0);
+    const int8_t* other = (const int8_t*)(color ? d.c0 : d.c1);
+    const u32 minus = int8_update_chunk<NDIM, HB, V, true>(own, other, d.g, d.tb, t, color, d.k0,
+                                                            d.k1, ci);
+    return (u32)V - minus;
+}
+
+template <int NDIM, int HB>
+__device__ __forceinline__ u32 batch_int8_v(const BatchDesc& d, u32 local, u32 t, u32 color)
+{
+    if (d.vec == 16) return batch_int8_tile<NDIM, HB, 16>(d, local, t, color);
+    if (d.vec == 8) return batch_int8_tile<NDIM, HB, 8>(d, local, t, color);
+    return batch_int8_tile<NDIM, HB, 4>(d, local, t, color);
+}
+
+__global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
+{
+    __shared__ MscCompactLds<2> lds;
+    cg::grid_group grid = cg::this_grid();
+    const u32 done0 = (u32)*ka.sweeps_done;
+    const u32 gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 gthreads = gridDim.x * blockDim.x;
+    for (u32 s = 0; s < ka.n_sweeps; ++s) {
+        const u32 sw = done0 + s;
+        const u32 par = sw & 1u;
+        for (u32 color = 0; color < 2; ++color) {
+            for (u32 tile = blockIdx.x; tile < ka.total_tiles; tile += gridDim.x) {
+                const u32 lat = ka.tile_lat[tile];
+                const BatchDesc& d = ka.descs[lat];
+                const u32 local = tile - d.tile_begin;
+                const u32 t = d.t0 + sw;
+                u32 plus = 0;
+                if (d.kind == BATCH_INT8) {
+                    if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1>(d, local, t, color)
+                                                 : batch_int8_v<2, 0>(d, local, t, color);
+                    else plus = d.hb ? batch_int8_v<3, 1>(d, local, t, color)
+                                     : batch_int8_v<3, 0>(d, local, t, color);
+                } else {
+                    MscArgs a;
+                    a.own = (u64*)(color ? d.c1 : d.c0);
+                    a.other = (const u64*)(color ? d.c0 : d.c1);
+                    a.rows = d.rows;
+                    a.nw = d.nw;
+                    a.n_words = d.n_words;
+                    a.div_nw = d.div_nw;
+                    a.t = t;
+                    a.color = color;
+                    a.k0 = d.k0;
+                    a.k1 = d.k1;
+                    u32 calls = 0, slots = 0, words = 0;
+                    if (ka.msc_mode == 0) {
+                        plus = msc_compact_tile<2, false, true>(a, d.codes, ka.msc_cap, local * 512u,
+                                                                lds, calls, slots, words);
+                    } else {
+                        const u32 w = local * 256u + threadIdx.x;
+                        MscState st = {};
+                        if (w < a.n_words) st = msc_classify(a, w);
+                        msc_run<true, false>(a, d.codes, w, 0u, 16u, st, calls, slots);
+                        if (w < a.n_words) {
+                            a.own[w] = (u64)st.pwl | ((u64)st.pwh << 32);
+                            plus = __popc(st.pwl) + __popc(st.pwh);
+                        }
+                    }
+                }
+                block_store_sum(ka.plus + ((size_t)par * 2 + color) * ka.total_tiles + tile, plus);
+            }
+            grid.sync();
+        }
+        // M of every lattice, one wave each; this parity's slots are next written two grid
+        // barriers later, so the next sweep can start right away.
+        const u32 gwave = gtid >> 6, gwaves = gthreads >> 6;
+        for (u32 lat = gwave; lat < ka.n_lat; lat += gwaves) batch_accumulate_lattice(ka, lat, par);
+    }
+    grid.sync();
+    if (gtid == 0) *ka.sweeps_done = done0 + ka.n_sweeps;
+}
+
+// ------------------------------------------------------------- launch-per-half-sweep form ----
+// The same tiles, one workgroup per tile and one launch per half sweep: the kernel boundary is
+// the barrier between colours. Sweep index = sweeps_done (device) so captured graphs replay.
+__global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
+{
+    __shared__ MscCompactLds<2> lds;
+    const u32 tile = xcd_band_block(blockIdx.x, gridDim.x);
+    const u32 sw = (u32)*ka.sweeps_done;
+    const u32 par = sw & 1u;
+    const u32 lat = ka.tile_lat[tile];
+    const BatchDesc& d = ka.descs[lat];
+    const u32 local = tile - d.tile_begin;
+    const u32 t = d.t0 + sw;
+    u32 plus = 0;
+    if (d.kind == BATCH_INT8) {
+        if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1>(d, local, t, color)
+                                     : batch_int8_v<2, 0>(d, local, t, color);
+        else plus = d.hb ? batch_int8_v<3, 1>(d, local, t, color)
+                         : batch_int8_v<3, 0>(d, local, t, color);
+    } else {
+        MscArgs a;
+        a.own = (u64*)(color ? d.c1 : d.c0);
+        a.other = (const u64*)(color ? d.c0 : d.c1);
+        a.rows = d.rows;
+        a.nw = d.nw;
+        a.n_words = d.n_words;
+        a.div_nw = d.div_nw;
+        a.t = t;
+        a.color = color;
+        a.k0 = d.k0;
+        a.k1 = d.k1;
+        u32 calls = 0, slots = 0, words = 0;
+        if (ka.msc_mode == 0) {
+            plus = msc_compact_tile<2, false, true>(a, d.codes, ka.msc_cap, local * 512u, lds, calls,
+                                                    slots, words);
+        } else {
+            const u32 w = local * 256u + threadIdx.x;
+            MscState st = {};
+            if (w < a.n_words) st = msc_classify(a, w);
+            msc_run<true, false>(a, d.codes, w, 0u, 16u, st, calls, slots);
+            if (w < a.n_words) {
+                a.own[w] = (u64)st.pwl | ((u64)st.pwh << 32);
+                plus = __popc(st.pwl) + __popc(st.pwh);
+            }
+        }
+    }
+    block_store_sum(ka.plus + ((size_t)par * 2 + color) * ka.total_tiles + tile, plus);
+}
+
+// After both colours of a sweep, one workgroup of 1024 threads: a wave per lattice accumulates,
+// then the sweep counter advances.
+__global__ __launch_bounds__(1024) void batch_accumulate(BatchKernelArgs ka)
+{
+    const u32 sw = (u32)*ka.sweeps_done;
+    for (u32 lat = threadIdx.x >> 6; lat < ka.n_lat; lat += blockDim.x >> 6)
+        batch_accumulate_lattice(ka, lat, sw & 1u);
+    __syncthreads();  // every thread has read the counter
+    if (threadIdx.x == 0) *ka.sweeps_done = sw + 1u;
+}
+
+__global__ void batch_zero_u64(unsigned long long* p, u32 n)
+{
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------- host side ----
+struct BatchHandle {
+    u32 n_lat = 0, total_tiles = 0;
+    BatchDesc* d_descs = nullptr;
+    u32* d_tile_lat = nullptr;
+    u32* d_plus = nullptr;
+    BatchAcc* d_acc = nullptr;
+    unsigned long long* d_done = nullptr;
+    int grid = 0, max_blocks = 0;
+    u32 msc_mode = 0, msc_cap = 4;
+    int persistent = 0;  // 1: one cooperative launch per call; 0: three launches per sweep
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    long long graph_sweeps = 0;
+};
+
+static void batch_free(BatchHandle* h)
+{
+    if (!h) return;
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    (void)hipFree(h->d_descs);
+    (void)hipFree(h->d_tile_lat);
+    (void)hipFree(h->d_plus);
+    (void)hipFree(h->d_acc);
+    (void)hipFree(h->d_done);
+    delete h;
+}
+
+static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
+{
+    BatchKernelArgs ka;
+    ka.descs = h->d_descs;
+    ka.tile_lat = h->d_tile_lat;
+    ka.plus = h->d_plus;
+    ka.acc = h->d_acc;
+    ka.sweeps_done = h->d_done;
+    ka.n_lat = h->n_lat;
+    ka.total_tiles = h->total_tiles;
+    ka.n_sweeps = n_sweeps;
+    ka.msc_mode = h->msc_mode;
+    ka.msc_cap = h->msc_cap;
+    if (h->persistent) {
+        void* args[] = {&ka};
+        ISING_HIP(hipLaunchCooperativeKernel((const void*)batch_persistent, dim3(h->grid),
+                                             dim3(256), args, 0, st));
+        return ISING_OK;
+    }
+    for (u32 s = 0; s < n_sweeps; ++s) {
+        hipLaunchKernelGGL(batch_half, dim3(h->total_tiles), dim3(256), 0, st, ka, 0u);
+        hipLaunchKernelGGL(batch_half, dim3(h->total_tiles), dim3(256), 0, st, ka, 1u);
+        hipLaunchKernelGGL(batch_accumulate, dim3(1), dim3(1024), 0, st, ka);
+    }
+    ISING_LAUNCHED("batch_half");
+    return ISING_OK;
+}
+
+extern "C" {
+
+typedef struct {
+    int kind;       // 0 = int8, 1 = msc
+    int ndim;       // 2 or 3 (msc: 2)
+    int heatbath;   // int8 only
+    int lz, ly, lx; // lattice shape, lz = 1 in 2D
+    unsigned long long seed;
+    uint32_t table[7];  // thresholds, 2 * ndim + 1 used (msc: table[3], table[4])
+    void* c0;       // device colour arrays, caller-owned
+    void* c1;
+    uint32_t t0;    // sweep counter of the lattice when the batch starts
+} IsingBatchLattice;
+
+int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, int msc_mode,
+                       int msc_cap, void** out)
+{
+    if (!lats || n <= 0 || !out) return ising_fail(ISING_EINVAL, "batch_create: bad argument");
+    if (msc_mode != 0 && msc_mode != 1) return ising_fail(ISING_EINVAL, "batch_create: msc_mode must be 0 or 1");
+    if (msc_cap < 1 || msc_cap > 15) return ising_fail(ISING_EINVAL, "batch_create: msc_cap in 1..15");
+    *out = nullptr;
+    std::vector<BatchDesc> descs((size_t)n);
+    std::vector<u32> tile_lat;
+    u32 tiles = 0;
+    for (int i = 0; i < n; ++i) {
+        const IsingBatchLattice& L = lats[i];
+        BatchDesc& d = descs[(size_t)i];
+        d = BatchDesc{};
+        if (!L.c0 || !L.c1) return ising_fail(ISING_EINVAL, "batch_create: null colour array");
+        if (L.lz < 1 || L.ly < 2 || L.lx < 8 || (L.ly & 1) || (L.lx & 1) || (L.ndim == 3 && (L.lz < 2 || (L.lz & 1))))
+            return ising_fail(ISING_EINVAL, "batch_create: bad shape");
+        d.c0 = L.c0;
+        d.c1 = L.c1;
+        d.kind = (u32)L.kind;
+        d.ndim = (u32)L.ndim;
+        d.hb = (u32)(L.heatbath != 0);
+        d.k0 = (u32)(L.seed & 0xFFFFFFFFull);
+        d.k1 = (u32)(L.seed >> 32);
+        d.t0 = L.t0;
+        d.n_spins = (long long)L.lz * L.ly * L.lx;
+        const long long rows = (long long)L.lz * L.ly;
+        if (L.kind == BATCH_INT8) {
+            if (L.ndim != 2 && L.ndim != 3) return ising_fail(ISING_EINVAL, "batch_create: int8 ndim 2 or 3");
+            if (L.ndim == 2 && L.lz != 1) return ising_fail(ISING_EINVAL, "batch_create: 2D needs lz = 1");
+            const u32 lh = (u32)(L.lx / 2);
+            const u32 v = (lh % 16 == 0) ? 16u : (lh % 8 == 0) ? 8u : (lh % 4 == 0) ? 4u : 0u;
+            if (!v) return ising_fail(ISING_EINVAL, "batch_create: int8 needs Lx % 8 == 0");
+            d.vec = v;
+            d.g.lh = lh;
+            d.g.ly = (u32)L.ly;
+            d.g.lz = (u32)L.lz;
+            d.g.chunks_row = lh / v;
+            d.g.n_chunks = (u32)(rows * (lh / v));
+            d.g.div_chunks = make_fastdiv(d.g.chunks_row);
+            d.g.div_ly = make_fastdiv(d.g.ly);
+            for (int k = 0; k <= 2 * L.ndim; ++k) d.tb.v[k] = L.table[k];
+            d.n_tiles = (d.g.n_chunks + 255u) / 256u;
+        } else if (L.kind == BATCH_MSC) {
+            if (L.ndim != 2 || L.lz != 1 || L.lx % 128) return ising_fail(ISING_EINVAL, "batch_create: msc needs 2D, Lx % 128 == 0");
+            d.rows = (u32)L.ly;
+            d.nw = (u32)(L.lx / 128);
+            d.n_words = d.rows * d.nw;
+            d.div_nw = make_fastdiv(d.nw);
+            d.codes = msc_level_codes(L.table[3], L.table[4]);
+            const u32 per = msc_mode == 0 ? 512u : 256u;
+            d.n_tiles = (d.n_words + per - 1u) / per;
+        } else {
+            return ising_fail(ISING_EINVAL, "batch_create: kind must be 0 (int8) or 1 (msc)");
+        }
+        d.tile_begin = tiles;
+        tiles += d.n_tiles;
+        for (u32 k = 0; k < d.n_tiles; ++k) tile_lat.push_back((u32)i);
+    }
+    int dev = 0, n_cu = 0, max_per_cu = 0;
+    ISING_HIP(hipGetDevice(&dev));
+    ISING_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    ISING_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&max_per_cu, (const void*)batch_persistent, 256, 0));
+    if (max_per_cu < 1) return ising_fail(ISING_EHIP, "batch_create: kernel does not fit on a CU");
+    const int per_cu = (blocks_per_cu <= 0 || blocks_per_cu > max_per_cu) ? max_per_cu : blocks_per_cu;
+
+    BatchHandle* h = new BatchHandle();
+    h->n_lat = (u32)n;
+    h->total_tiles = tiles;
+    h->grid = per_cu * n_cu;
+    if ((u32)h->grid > tiles) h->grid = (int)tiles;
+    h->max_blocks = max_per_cu;
+    h->msc_mode = (u32)msc_mode;
+    h->msc_cap = (u32)msc_cap;
+    hipError_t e = hipMalloc((void**)&h->d_descs, sizeof(BatchDesc) * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->d_tile_lat, sizeof(u32) * tile_lat.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&h->d_plus, sizeof(u32) * 4 * (size_t)tiles);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->d_acc, sizeof(BatchAcc) * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->d_done, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemcpy(h->d_descs, descs.data(), sizeof(BatchDesc) * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_tile_lat, tile_lat.data(), sizeof(u32) * tile_lat.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(256), 0, 0, (unsigned long long*)h->d_acc,
+                           (u32)(sizeof(BatchAcc) / 8) * (u32)n);
+        hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(64), 0, 0, h->d_done, 1u);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        batch_free(h);
+        return ising_check_hip(e, "batch_create");
+    }
+    *out = h;
+    return ISING_OK;
+}
+
+// n_sweeps sweeps of every lattice, one cooperative launch (stream-ordered, asynchronous).
+int ising_batch_sweeps(void* handle, long long n_sweeps, void* stream)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || n_sweeps < 0 || n_sweeps > 0x7FFFFFFFll) return ising_fail(ISING_EINVAL, "batch_sweeps: bad argument");
+    if (n_sweeps == 0) return ISING_OK;
+    return batch_launch(h, (u32)n_sweeps, as_stream(stream));
+}
+
+// Capture one launch of `sweeps_per_replay` sweeps into a hipGraph (stream capture); replays
+// continue the device-side sweep counter. Returns ISING_EHIP if capture is not supported for
+// the cooperative launch on this runtime (callers then use ising_batch_sweeps).
+int ising_batch_graph(void* handle, long long sweeps_per_replay, void* stream)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || sweeps_per_replay <= 0 || sweeps_per_replay > 0x7FFFFFFFll)
+        return ising_fail(ISING_EINVAL, "batch_graph: bad argument");
+    hipStream_t st = as_stream(stream);
+    if (!st) return ising_fail(ISING_EINVAL, "batch_graph: capture needs a created stream");
+    if (h->graph_exec) {
+        (void)hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+    }
+    if (h->graph) {
+        (void)hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    ISING_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = batch_launch(h, (u32)sweeps_per_replay, st);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return ising_check_hip(e, "hipStreamEndCapture");
+    h->graph = g;
+    ISING_HIP(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+    h->graph_sweeps = sweeps_per_replay;
+    return ISING_OK;
+}
+
+int ising_batch_replay(void* handle, int replays, void* stream)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || !h->graph_exec || replays < 0) return ising_fail(ISING_EINVAL, "batch_replay: no graph");
+    for (int r = 0; r < replays; ++r) ISING_HIP(hipGraphLaunch(h->graph_exec, as_stream(stream)));
+    return ISING_OK;
+}
+
+// Accumulators: n x 6 int64 (sum M, sum |M|, sum M^2, sum M^4 lo, sum M^4 hi, samples) and the
+// number of sweeps done. Synchronous.
+int ising_batch_stats(void* handle, long long* out, long long* sweeps_done, void* stream)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || !out) return ising_fail(ISING_EINVAL, "batch_stats: bad argument");
+    ISING_HIP(hipStreamSynchronize(as_stream(stream)));
+    ISING_HIP(hipMemcpy(out, h->d_acc, sizeof(BatchAcc) * h->n_lat, hipMemcpyDeviceToHost));
+    if (sweeps_done) {
+        unsigned long long d = 0;
+        ISING_HIP(hipMemcpy(&d, h->d_done, sizeof(d), hipMemcpyDeviceToHost));
+        *sweeps_done = (long long)d;
+    }
+    return ISING_OK;
+}
+
+int ising_batch_reset_stats(void* handle, void* stream)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h) return ising_fail(ISING_EINVAL, "batch_reset_stats: bad handle");
+    hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(256), 0, as_stream(stream),
+                       (unsigned long long*)h->d_acc, (u32)(sizeof(BatchAcc) / 8) * h->n_lat);
+    ISING_LAUNCHED("batch_zero_u64");
+    return ISING_OK;
+}
+
+// 1 = one persistent cooperative launch per call (grid barrier between half sweeps),
+// 0 = one launch per half sweep plus one accumulation launch per sweep (the default).
+// Both give identical lattices and sums. Destroys a captured graph.
+int ising_batch_set_persistent(void* handle, int persistent)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || (persistent != 0 && persistent != 1))
+        return ising_fail(ISING_EINVAL, "batch_set_persistent: bad argument");
+    if (h->graph_exec) {
+        (void)hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+    }
+    if (h->graph) {
+        (void)hipGraphDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    h->graph_sweeps = 0;
+    h->persistent = persistent;
+    return ISING_OK;
+}
+
+int ising_batch_info(void* handle, int* grid, int* total_tiles, int* max_blocks_per_cu)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h) return ising_fail(ISING_EINVAL, "batch_info: bad handle");
+    if (grid) *grid = h->grid;
+    if (total_tiles) *total_tiles = (int)h->total_tiles;
+    if (max_blocks_per_cu) *max_blocks_per_cu = h->max_blocks;
+    return ISING_OK;
+}
+
+int ising_batch_destroy(void* handle)
+{
+    batch_free((BatchHandle*)handle);
+    return ISING_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+"""Batched temperature scans: every lattice of a persistent batched launch is bit-identical to
+the same lattice swept alone, and the device-side Binder sums are exact."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import ising_oracle as O
+
+
+def _mixed(hip, seed0=100):
+    """A batch covering every tile kind: int8 2D/3D x Metropolis/heat bath, msc of several
+    word counts, uneven sizes and temperatures."""
+    specs = [
+        (hip.HipInt8, (64, 64), 1 / 2.2, "metropolis"),
+        (hip.HipInt8, (32, 48), 1 / 2.3, "heatbath"),
+        (hip.HipInt8, (16, 24), 0.42, "metropolis"),  # V = 4
+        (hip.HipInt8, (8, 8, 16), 0.22, "metropolis"),
+        (hip.HipInt8, (6, 4, 8), 0.24, "heatbath"),
+        (hip.HipMultiSpin, (128, 256), 0.44, "metropolis"),
+        (hip.HipMultiSpin, (64, 384), 1 / 2.25, "metropolis"),
+        (hip.HipMultiSpin, (130, 128), 0.5, "metropolis"),
+        (hip.HipInt8, (256, 256), 1 / 2.269, "metropolis"),
+        (hip.HipMultiSpin, (1024, 1024), 1 / 2.269, "metropolis"),
+    ]
+    out = []
+    for i, (cls, shape, beta, alg) in enumerate(specs):
+        lat = O.random_lattice(shape, 7 + i)
+        out.append((cls, shape, beta, alg, seed0 + i, lat))
+    return out
+
+
+def _make(spec):
+    cls, shape, beta, alg, seed, lat = spec
+    return cls(shape, beta, seed=seed, algorithm=alg, init=lat)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from pyising_amd.backends import hip_backend
+
+    return hip_backend
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("persistent", [False, True])
+@pytest.mark.parametrize("msc_mode", [0, 1])
+def test_batch_bitwise_equals_standalone(hip, msc_mode, persistent):
+    from pyising_amd.batch import BatchScan
+
+    specs = _mixed(hip)
+    sims = [_make(s) for s in specs]
+    refs = [_make(s) for s in specs]
+    scan = BatchScan(sims, msc_mode=msc_mode, persistent=persistent)
+    scan.sweep(3)
+    scan.sweep(4)
+    scan.synchronize()
+    for sim, ref in zip(sims, refs, strict=True):
+        ref.sweep(7)
+        assert sim.t == ref.t == 7
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    scan.close()
+
+
+@pytest.mark.gpu
+def test_batch_matches_oracle(hip):
+    from pyising_amd.batch import BatchScan
+
+    specs = _mixed(hip)[:8]
+    sims = [_make(s) for s in specs]
+    scan = BatchScan(sims, blocks_per_cu=2)
+    scan.sweep(5)
+    scan.synchronize()
+    for spec, sim in zip(specs, sims, strict=True):
+        cls, shape, beta, alg, seed, lat = spec
+        ref = (O.MscOracle if cls is hip.HipMultiSpin else O.Int8Oracle)(
+            shape, beta, seed=seed, algorithm=alg, init=lat)
+        ref.sweep(5)
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    scan.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_moments_are_exact(hip, persistent):
+    from pyising_amd.batch import BatchScan
+
+    specs = _mixed(hip)
+    sims = [_make(s) for s in specs]
+    refs = [_make(s) for s in specs]
+    scan = BatchScan(sims, persistent=persistent)
+    n = 9
+    scan.sweep(n)
+    mom = scan.moments()
+    assert scan.sweeps_done == n
+    for ref, mo in zip(refs, mom, strict=True):
+        ms = []
+        for _ in range(n):
+            ref.sweep(1)
+            ms.append(ref.observables_exact()[0])
+        ms = [int(m) for m in ms]
+        assert mo.samples == n
+        assert mo.sum_m == sum(ms)
+        assert mo.sum_abs_m == sum(abs(m) for m in ms)
+        assert mo.sum_m2 == sum(m * m for m in ms)
+        assert mo.sum_m4 == sum(m**4 for m in ms)
+    scan.reset_moments()
+    assert all(m.samples == 0 for m in scan.moments())
+    scan.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_graph_replays_continue_the_stream(hip, persistent):
+    from pyising_amd.batch import BatchScan
+
+    specs = _mixed(hip)[:6]
+    sims = [_make(s) for s in specs]
+    refs = [_make(s) for s in specs]
+    scan = BatchScan(sims, persistent=persistent)
+    if not scan.capture(2):
+        scan.close()
+        pytest.skip("the runtime cannot capture a cooperative launch into a graph")
+    scan.replay(3)
+    scan.sweep(1)  # a direct launch after replays continues the same counter
+    scan.synchronize()
+    for sim, ref in zip(sims, refs, strict=True):
+        ref.sweep(7)
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    assert scan.moments()[0].samples == 7
+    scan.close()
+
+
+@pytest.mark.gpu
+def test_batch_rejects_bad_input(hip):
+    from pyising_amd import _lib
+    from pyising_amd.batch import BatchScan
+
+    with pytest.raises(ValueError):
+        BatchScan([])
+    eager = hip.HipMultiSpinEager((128, 128), 0.4, seed=1)
+    with pytest.raises(ValueError):
+        BatchScan([eager])
+    with pytest.raises(_lib.HipError):
+        BatchScan([hip.HipInt8((16, 16), 0.4)], msc_cap=0)
+
+
+# ------------------------------------------------------------------------ CPU (host logic) --
+def test_struct_layout_matches_header():
+    from pyising_amd.batch import IsingBatchLattice
+
+    assert C.sizeof(IsingBatchLattice) == 88
+    assert IsingBatchLattice.seed.offset == 24
+    assert IsingBatchLattice.table.offset == 32
+    assert IsingBatchLattice.c0.offset == 64
+    assert IsingBatchLattice.t0.offset == 80
+
+
+def test_moments_math():
+    from pyising_amd.batch import Moments
+
+    ms = np.array([4, -4, 2, -2, 4], dtype=np.int64)
+    mo = Moments(samples=5, sum_m=int(ms.sum()), sum_abs_m=int(np.abs(ms).sum()),
+                 sum_m2=int((ms**2).sum()), sum_m4=int((ms**4).sum()), n_spins=4)
+    m = ms / 4.0
+    assert mo.mean(1) == pytest.approx(m.mean())
+    assert mo.abs_m == pytest.approx(np.abs(m).mean())
+    assert mo.binder == pytest.approx(1 - (m**4).mean() / (3 * (m**2).mean() ** 2))
+    ordered = Moments(samples=3, sum_m=3 * 16, sum_abs_m=3 * 16, sum_m2=3 * 256,
+                      sum_m4=3 * 65536, n_spins=16)
+    assert ordered.binder == pytest.approx(2 / 3)
+    empty = Moments(0, 0, 0, 0, 0, 16)
+    assert np.isnan(empty.binder)
