@@ -66,6 +66,9 @@ int ising_event_destroy(void* ev);
 int ising_event_record(void* ev, void* stream);
 int ising_event_query(void* ev); /* 1 = complete, 0 = pending, < 0 error */
 int ising_event_elapsed_ms(float* ms, void* start, void* stop);
+int ising_event_sync(void* ev);
+/* Work queued on `stream` after this call waits for the last record of `ev`. */
+int ising_stream_wait_event(void* stream, void* ev);
 
 /* ----------------------------------------------------------------- layouts ---- */
 /* Full lattice (rows, lx) bytes -> two int8 colour arrays (rows, lx/2); any byte > 0 is +1,
@@ -173,6 +176,18 @@ int ising_batch_info(void* handle, int* grid, int* total_tiles, int* max_blocks_
  * sweep; 1 = one persistent cooperative launch with a grid barrier between half sweeps. */
 int ising_batch_set_persistent(void* handle, int persistent);
 int ising_batch_destroy(void* handle);
+/* Tiles of one half sweep of these lattices (host only, no device call): the tile capacity a
+ * reserved batch needs for them. */
+int ising_batch_tiles(const IsingBatchLattice* lats, int n, int msc_mode, long long* tiles);
+/* Continuous batching (no reference counterpart): a batch with device tables for up to
+ * max_lattices lattices and max_tiles tiles and no lattices yet. Allocates once (synchronous). */
+int ising_batch_reserve(int max_lattices, long long max_tiles, int blocks_per_cu, int msc_mode,
+                        int msc_cap, void** handle);
+/* Replace the lattices of a reserved batch (n >= 0), stream-ordered on `stream` (pinned,
+ * double-buffered uploads; allocates nothing). Sweeps queued before see the old list, sweeps
+ * after it the new one; accumulators and the batch sweep counter restart at zero, lattice i
+ * continues from its own t0. ISING_ERANGE beyond the reserved capacity. */
+int ising_batch_assign(void* handle, const IsingBatchLattice* lats, int n, void* stream);
 
 /* ------------------------------------------------------------------ probes ---- */
 /* Philox4x32-10 of n (counter[4], key[2]) pairs, host arrays in and out: the known-answer

@@ -41,6 +41,8 @@ SIGNATURES: dict[str, list] = {
     "ising_event_record": [_P, _P],
     "ising_event_query": [_P],
     "ising_event_elapsed_ms": [_P, _P, _P],
+    "ising_event_sync": [_P],
+    "ising_stream_wait_event": [_P, _P],
     "ising_philox_kat": [_P, _P, _P, _I],
     "ising_philox_rate": [_P, _I, _I, _U32, _U32, _P],
     "ising_copy": [_P, _P, _SZ, _I, _P],
@@ -67,6 +69,9 @@ SIGNATURES: dict[str, list] = {
     "ising_batch_info": [_P, _P, _P, _P],
     "ising_batch_set_persistent": [_P, _I],
     "ising_batch_destroy": [_P],
+    "ising_batch_tiles": [_P, _I, _I, _P],
+    "ising_batch_reserve": [_I, _LL, _I, _I, _I, _P],
+    "ising_batch_assign": [_P, _P, _I, _P],
 }
 
 
@@ -159,6 +164,25 @@ class DeviceBuffer:
         check(self._lib.ising_memcpy_d2h(host.ctypes.data, self.ptr, host.nbytes, stream), "d2h")
 
 
+class DeviceView(DeviceBuffer):
+    """A range of device memory owned elsewhere (a sub-allocation); ``free`` releases nothing
+    but the view, or hands the range back through ``release``."""
+
+    __slots__ = ("_release",)
+
+    def __init__(self, ptr: int, nbytes: int, release=None):  # noqa: D107 - no allocation
+        self._lib = lib()
+        self.ptr = int(ptr)
+        self.nbytes = int(nbytes)
+        self._release = release
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            self.ptr = None
+            if self._release is not None:
+                self._release()
+
+
 class Event:
     """HIP event (timing on a given stream)."""
 
@@ -179,6 +203,9 @@ class Event:
     def done(self) -> bool:
         return check(self._lib.ising_event_query(self.ptr), "event_query") == 1
 
+    def sync(self) -> None:
+        check(self._lib.ising_event_sync(self.ptr), "event_sync")
+
     def __del__(self):
         try:
             if self.ptr:
@@ -198,6 +225,10 @@ class Stream:
 
     def sync(self) -> None:
         check(self._lib.ising_stream_sync(self.ptr), "stream_sync")
+
+    def wait(self, event: Event) -> None:
+        """Later work on this stream waits for the last record of ``event``."""
+        check(self._lib.ising_stream_wait_event(self.ptr, event.ptr), "stream_wait_event")
 
     def __del__(self):
         try:

@@ -34,15 +34,54 @@ def _full_lattice(lat: np.ndarray, shape) -> np.ndarray:
 
 
 class _HipBase(IsingSimulation):
+    """Keyword-only options beyond the reference constructor (for servers that manage memory
+    and streams themselves; the defaults give the plain drop-in behaviour):
+
+    ``stream``: a created HIP stream (pointer) for every launch and copy of this simulation
+    (default: the null stream). ``allocator``: callable ``nbytes -> device buffer`` for the
+    resident state (colour arrays, reduction scratch; default: one hipMalloc each).
+    ``staging``: a device buffer reused for host uploads of full lattices when it is large
+    enough (default: a temporary allocation per upload).
+    """
+
     backend = "hip"
     gpu = True
+
+    def __init__(self, shape, beta: float, seed: int = 0, algorithm: str = "metropolis",
+                 init="hot", *, stream=None, allocator=None, staging=None) -> None:
+        self.stream_ptr = stream
+        self._allocator = allocator
+        self._staging = staging
+        super().__init__(shape, beta, seed=seed, algorithm=algorithm, init=init)
+
+    def _alloc(self, nbytes: int):
+        return self._allocator(nbytes) if self._allocator else _lib.DeviceBuffer(nbytes)
+
+    def _full_buffer(self, nbytes: int):
+        """A device buffer for one full lattice: (staging or a new allocation, temporary?)."""
+        if self._staging is not None and self._staging.nbytes >= nbytes:
+            return self._staging, False
+        return _lib.DeviceBuffer(nbytes), True
+
+    def _upload_full(self, full: np.ndarray):
+        """Full host lattice -> device; returns (buffer, temporary?)."""
+        buf, tmp = self._full_buffer(full.nbytes)
+        buf.upload(full, self.stream_ptr)
+        return buf, tmp
 
     def _hip_setup(self) -> None:
         self._L = _lib.lib()
         self.k0, self.k1 = split_seed(self.seed)
-        self.stream_ptr = None  # the null stream, like the reference's default-stream launches
-        self._obs_scratch = _lib.DeviceBuffer(16)
-        self._obs_out = _lib.DeviceBuffer(16)
+        self._obs_scratch = self._alloc(16)
+        self._obs_out = self._alloc(16)
+
+    @staticmethod
+    def state_bytes(shape, variant: str) -> list[int]:
+        """Sizes of the device allocations a simulation of ``shape`` makes through its
+        allocator: two colour arrays and two 16-byte reduction buffers."""
+        n = int(np.prod(shape))
+        colour = n // 16 if variant == "msc" else n // 2
+        return [16, 16, colour, colour]
 
     def synchronize(self) -> None:
         _lib.check(self._L.ising_stream_sync(self.stream_ptr), "stream_sync")
@@ -63,20 +102,20 @@ class HipInt8(_HipBase):
         self.ly = self.shape[-2]
         self.lz = self.shape[0] if self.ndim == 3 else 1
         nbytes = self.rows * self.lh
-        self.c = [_lib.DeviceBuffer(nbytes), _lib.DeviceBuffer(nbytes)]
+        self.c = [self._alloc(nbytes), self._alloc(nbytes)]
         self._table_arr = np.ascontiguousarray(self.table, dtype=np.uint32)
 
     def _upload_table(self) -> None:
         self._table_arr = np.ascontiguousarray(self.table, dtype=np.uint32)
 
     def _split_from_host(self, full: np.ndarray) -> None:
-        tmp = _lib.DeviceBuffer(full.nbytes)
-        tmp.upload(full, self.stream_ptr)
-        _lib.check(self._L.ising_split_int8(tmp.ptr, self.c[0].ptr, self.c[1].ptr, self.rows,
+        buf, tmp = self._upload_full(full)
+        _lib.check(self._L.ising_split_int8(buf.ptr, self.c[0].ptr, self.c[1].ptr, self.rows,
                                             self.ly, self.shape[-1], self.stream_ptr),
                    "split_int8")
         self.synchronize()
-        tmp.free()
+        if tmp:
+            buf.free()
 
     def set_lattice(self, lat: np.ndarray) -> None:
         self._split_from_host(_full_lattice(lat, self.shape))
@@ -91,12 +130,13 @@ class HipInt8(_HipBase):
 
     def get_lattice(self) -> np.ndarray:
         full = np.empty(self.shape, dtype=np.int8)
-        tmp = _lib.DeviceBuffer(full.nbytes)
-        _lib.check(self._L.ising_merge_int8(self.c[0].ptr, self.c[1].ptr, tmp.ptr, self.rows,
+        buf, tmp = self._full_buffer(full.nbytes)
+        _lib.check(self._L.ising_merge_int8(self.c[0].ptr, self.c[1].ptr, buf.ptr, self.rows,
                                             self.ly, self.shape[-1], self.stream_ptr),
                    "merge_int8")
-        tmp.download(full, self.stream_ptr)
-        tmp.free()
+        buf.download(full, self.stream_ptr)
+        if tmp:
+            buf.free()
         return full
 
     def _half_sweep(self, color: int) -> None:
@@ -140,16 +180,16 @@ class HipMultiSpin(_HipBase):
         self._hip_setup()
         self.nw = self.lh // 64
         self.n_words = self.rows * self.nw
-        self.w = [_lib.DeviceBuffer(8 * self.n_words), _lib.DeviceBuffer(8 * self.n_words)]
+        self.w = [self._alloc(8 * self.n_words), self._alloc(8 * self.n_words)]
         self.stats_buf = None
 
     def _split_from_host(self, full: np.ndarray) -> None:
-        tmp = _lib.DeviceBuffer(full.nbytes)
-        tmp.upload(full, self.stream_ptr)
-        _lib.check(self._L.ising_split_pack(tmp.ptr, self.w[0].ptr, self.w[1].ptr, self.rows,
+        buf, tmp = self._upload_full(full)
+        _lib.check(self._L.ising_split_pack(buf.ptr, self.w[0].ptr, self.w[1].ptr, self.rows,
                                             self.shape[-1], self.stream_ptr), "split_pack")
         self.synchronize()
-        tmp.free()
+        if tmp:
+            buf.free()
 
     def set_lattice(self, lat: np.ndarray) -> None:
         self._split_from_host(_full_lattice(lat, self.shape))
@@ -164,11 +204,12 @@ class HipMultiSpin(_HipBase):
 
     def get_lattice(self) -> np.ndarray:
         full = np.empty(self.shape, dtype=np.int8)
-        tmp = _lib.DeviceBuffer(full.nbytes)
-        _lib.check(self._L.ising_unpack_merge(self.w[0].ptr, self.w[1].ptr, tmp.ptr, self.rows,
+        buf, tmp = self._full_buffer(full.nbytes)
+        _lib.check(self._L.ising_unpack_merge(self.w[0].ptr, self.w[1].ptr, buf.ptr, self.rows,
                                               self.shape[-1], self.stream_ptr), "unpack_merge")
-        tmp.download(full, self.stream_ptr)
-        tmp.free()
+        buf.download(full, self.stream_ptr)
+        if tmp:
+            buf.free()
         return full
 
     def get_words(self) -> tuple[np.ndarray, np.ndarray]:

@@ -100,7 +100,7 @@ class BatchScan:
     """
 
     def __init__(self, sims, blocks_per_cu: int | None = None, msc_mode: int = 0,
-                 msc_cap: int = 4, persistent: bool = False):
+                 msc_cap: int = 4, persistent: bool = False, stream: _lib.Stream | None = None):
         self.sims = list(sims)
         if not self.sims:
             raise ValueError("empty batch")
@@ -115,7 +115,7 @@ class BatchScan:
                    "batch_set_persistent")
         self.persistent = bool(persistent)
         self.n_spins = np.array([s.n_spins for s in self.sims], dtype=np.int64)
-        self.stream = _lib.Stream()
+        self.stream = stream if stream is not None else _lib.Stream()
         self.graph_sweeps = 0
 
     def info(self) -> dict:
@@ -170,6 +170,69 @@ class BatchScan:
 
     def reset_moments(self) -> None:
         _lib.check(self._L.ising_batch_reset_stats(self._h, self.stream.ptr), "batch_reset")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.stream.sync()
+            self._L.ising_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter teardown
+            pass
+
+
+def _describe_all(sims) -> C.Array:
+    return (IsingBatchLattice * max(1, len(sims)))(*[_describe(s) for s in sims])
+
+
+def batch_tiles(sims, msc_mode: int = 0) -> int:
+    """Tiles of one half sweep of these simulations (host computation, no device call)."""
+    out = C.c_longlong()
+    _lib.check(_lib.load_library().ising_batch_tiles(_describe_all(sims), len(sims),
+                                                     int(msc_mode), C.byref(out)), "batch_tiles")
+    return out.value
+
+
+class ReservedBatch:
+    """A batch with fixed device capacity whose lattice list is replaced between sweeps
+    (continuous batching). ``assign`` and ``sweep`` are stream-ordered on ``stream`` and allocate
+    nothing; a sweep queued before an ``assign`` still runs the old list.
+    """
+
+    def __init__(self, max_lattices: int, max_tiles: int, stream: _lib.Stream, msc_mode: int = 0,
+                 msc_cap: int = 4, blocks_per_cu: int | None = None):
+        self._L = _lib.lib()
+        self.stream = stream
+        self.msc_mode = int(msc_mode)
+        h = C.c_void_p()
+        _lib.check(self._L.ising_batch_reserve(int(max_lattices), int(max_tiles),
+                                               int(blocks_per_cu or 0), self.msc_mode,
+                                               int(msc_cap), C.byref(h)), "batch_reserve")
+        self._h = h.value
+        self.max_lattices, self.max_tiles = int(max_lattices), int(max_tiles)
+        self.sims: list[IsingSimulation] = []
+
+    def assign(self, sims) -> None:
+        sims = list(sims)
+        _lib.check(self._L.ising_batch_assign(self._h, _describe_all(sims), len(sims),
+                                              self.stream.ptr), "batch_assign")
+        self.sims = sims
+
+    def sweep(self, n: int = 1) -> None:
+        n = int(n)
+        if n <= 0 or not self.sims:
+            return
+        _lib.check(self._L.ising_batch_sweeps(self._h, n, self.stream.ptr), "batch_sweeps")
+        for s in self.sims:
+            s.t += n
+
+    def info(self) -> dict:
+        g, t, m = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(self._L.ising_batch_info(self._h, C.byref(g), C.byref(t), C.byref(m)))
+        return {"grid": g.value, "tiles_per_half_sweep": t.value, "max_blocks_per_cu": m.value}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

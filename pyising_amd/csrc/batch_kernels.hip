@@ -26,6 +26,7 @@
 // the counter stream.
 #include <hip/hip_cooperative_groups.h>
 
+#include <cstring>
 #include <vector>
 
 #include "int8_core.hpp"
@@ -250,26 +251,60 @@ __global__ void batch_zero_u64(unsigned long long* p, u32 n)
 }
 
 // ---------------------------------------------------------------------- host side ----
+// Caller-side description of one lattice (the C ABI struct).
+typedef struct {
+    int kind;       // 0 = int8, 1 = msc
+    int ndim;       // 2 or 3 (msc: 2)
+    int heatbath;   // int8 only
+    int lz, ly, lx; // lattice shape, lz = 1 in 2D
+    unsigned long long seed;
+    uint32_t table[7];  // thresholds, 2 * ndim + 1 used (msc: table[3], table[4])
+    void* c0;       // device colour arrays, caller-owned
+    void* c1;
+    uint32_t t0;    // sweep counter of the lattice when the batch starts
+} IsingBatchLattice;
+
 struct BatchHandle {
     u32 n_lat = 0, total_tiles = 0;
+    u32 cap_lat = 0, cap_tiles = 0;  // device table capacity (reserved handles: > n_lat)
     BatchDesc* d_descs = nullptr;
     u32* d_tile_lat = nullptr;
     u32* d_plus = nullptr;
     BatchAcc* d_acc = nullptr;
     unsigned long long* d_done = nullptr;
-    int grid = 0, max_blocks = 0;
+    int grid = 0, max_blocks = 0, n_cu = 0, per_cu = 0;
     u32 msc_mode = 0, msc_cap = 4;
     int persistent = 0;  // 1: one cooperative launch per call; 0: three launches per sweep
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     long long graph_sweeps = 0;
+    // reserved handles: pinned staging of (descs, tile map), double-buffered; the event of a
+    // slot marks the end of its last upload, so the host never rewrites a slot in flight
+    void* pinned[2] = {nullptr, nullptr};
+    hipEvent_t pinned_done[2] = {nullptr, nullptr};
+    int next_slot = 0;
 };
+
+static void batch_drop_graph(BatchHandle* h)
+{
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_sweeps = 0;
+}
 
 static void batch_free(BatchHandle* h)
 {
     if (!h) return;
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    if (h->graph) (void)hipGraphDestroy(h->graph);
+    batch_drop_graph(h);
+    for (int s = 0; s < 2; ++s) {
+        if (h->pinned_done[s]) {
+            (void)hipEventSynchronize(h->pinned_done[s]);
+            (void)hipEventDestroy(h->pinned_done[s]);
+        }
+        if (h->pinned[s]) (void)hipHostFree(h->pinned[s]);
+    }
     (void)hipFree(h->d_descs);
     (void)hipFree(h->d_tile_lat);
     (void)hipFree(h->d_plus);
@@ -278,8 +313,110 @@ static void batch_free(BatchHandle* h)
     delete h;
 }
 
+// Host-side descriptors and tile map of n lattices (no device calls). On success `descs` holds
+// n entries and `tile_lat` one lattice index per tile.
+static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
+                       std::vector<BatchDesc>& descs, std::vector<u32>& tile_lat)
+{
+    descs.assign((size_t)n, BatchDesc{});
+    tile_lat.clear();
+    u32 tiles = 0;
+    for (int i = 0; i < n; ++i) {
+        const IsingBatchLattice& L = lats[i];
+        BatchDesc& d = descs[(size_t)i];
+        if (!L.c0 || !L.c1) return ising_fail(ISING_EINVAL, "batch: null colour array");
+        if (L.lz < 1 || L.ly < 2 || L.lx < 8 || (L.ly & 1) || (L.lx & 1) || (L.ndim == 3 && (L.lz < 2 || (L.lz & 1))))
+            return ising_fail(ISING_EINVAL, "batch: bad shape");
+        d.c0 = L.c0;
+        d.c1 = L.c1;
+        d.kind = (u32)L.kind;
+        d.ndim = (u32)L.ndim;
+        d.hb = (u32)(L.heatbath != 0);
+        d.k0 = (u32)(L.seed & 0xFFFFFFFFull);
+        d.k1 = (u32)(L.seed >> 32);
+        d.t0 = L.t0;
+        d.n_spins = (long long)L.lz * L.ly * L.lx;
+        const long long rows = (long long)L.lz * L.ly;
+        if (L.kind == BATCH_INT8) {
+            if (L.ndim != 2 && L.ndim != 3) return ising_fail(ISING_EINVAL, "batch: int8 ndim 2 or 3");
+            if (L.ndim == 2 && L.lz != 1) return ising_fail(ISING_EINVAL, "batch: 2D needs lz = 1");
+            const u32 lh = (u32)(L.lx / 2);
+            const u32 v = (lh % 16 == 0) ? 16u : (lh % 8 == 0) ? 8u : (lh % 4 == 0) ? 4u : 0u;
+            if (!v) return ising_fail(ISING_EINVAL, "batch: int8 needs Lx % 8 == 0");
+            if (rows * (lh / v) > 0xFFFFFFFFll) return ising_fail(ISING_ERANGE, "batch: int8 lattice too large");
+            d.vec = v;
+            d.g.lh = lh;
+            d.g.ly = (u32)L.ly;
+            d.g.lz = (u32)L.lz;
+            d.g.chunks_row = lh / v;
+            d.g.n_chunks = (u32)(rows * (lh / v));
+            d.g.div_chunks = make_fastdiv(d.g.chunks_row);
+            d.g.div_ly = make_fastdiv(d.g.ly);
+            for (int k = 0; k <= 2 * L.ndim; ++k) d.tb.v[k] = L.table[k];
+            d.n_tiles = (d.g.n_chunks + 255u) / 256u;
+        } else if (L.kind == BATCH_MSC) {
+            if (L.ndim != 2 || L.lz != 1 || L.lx % 128) return ising_fail(ISING_EINVAL, "batch: msc needs 2D, Lx % 128 == 0");
+            d.rows = (u32)L.ly;
+            d.nw = (u32)(L.lx / 128);
+            d.n_words = d.rows * d.nw;
+            d.div_nw = make_fastdiv(d.nw);
+            d.codes = msc_level_codes(L.table[3], L.table[4]);
+            const u32 per = msc_mode == 0 ? 512u : 256u;
+            d.n_tiles = (d.n_words + per - 1u) / per;
+        } else {
+            return ising_fail(ISING_EINVAL, "batch: kind must be 0 (int8) or 1 (msc)");
+        }
+        if ((unsigned long long)tiles + d.n_tiles > 0x7FFFFFFFull) return ising_fail(ISING_ERANGE, "batch: too many tiles");
+        d.tile_begin = tiles;
+        tiles += d.n_tiles;
+        for (u32 k = 0; k < d.n_tiles; ++k) tile_lat.push_back((u32)i);
+    }
+    return ISING_OK;
+}
+
+// Allocates the device tables for cap_lat lattices and cap_tiles tiles and sets the grid
+// parameters (synchronous).
+static int batch_alloc(BatchHandle* h, u32 cap_lat, u32 cap_tiles, int blocks_per_cu)
+{
+    int dev = 0;
+    ISING_HIP(hipGetDevice(&dev));
+    ISING_HIP(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    ISING_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&h->max_blocks, (const void*)batch_persistent, 256, 0));
+    if (h->max_blocks < 1) return ising_fail(ISING_EHIP, "batch: kernel does not fit on a CU");
+    h->per_cu = (blocks_per_cu <= 0 || blocks_per_cu > h->max_blocks) ? h->max_blocks : blocks_per_cu;
+    h->cap_lat = cap_lat;
+    h->cap_tiles = cap_tiles;
+    ISING_HIP(hipMalloc((void**)&h->d_descs, sizeof(BatchDesc) * (size_t)cap_lat));
+    ISING_HIP(hipMalloc((void**)&h->d_tile_lat, sizeof(u32) * (size_t)cap_tiles));
+    ISING_HIP(hipMalloc((void**)&h->d_plus, sizeof(u32) * 4 * (size_t)cap_tiles));
+    ISING_HIP(hipMalloc((void**)&h->d_acc, sizeof(BatchAcc) * (size_t)cap_lat));
+    ISING_HIP(hipMalloc((void**)&h->d_done, sizeof(unsigned long long)));
+    return ISING_OK;
+}
+
+static void batch_set_sizes(BatchHandle* h, u32 n_lat, u32 tiles)
+{
+    h->n_lat = n_lat;
+    h->total_tiles = tiles;
+    h->grid = h->per_cu * h->n_cu;
+    if ((u32)h->grid > tiles) h->grid = (int)tiles;
+}
+
+// Zero the accumulators of the first n_lat lattices and the sweep counter (kernels, so the
+// work is stream-ordered and capturable).
+static int batch_zero_stats(BatchHandle* h, hipStream_t st)
+{
+    if (h->n_lat)
+        hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(256), 0, st, (unsigned long long*)h->d_acc,
+                           (u32)(sizeof(BatchAcc) / 8) * h->n_lat);
+    hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(64), 0, st, h->d_done, 1u);
+    ISING_LAUNCHED("batch_zero_u64");
+    return ISING_OK;
+}
+
 static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
 {
+    if (h->total_tiles == 0) return ISING_OK;  // an empty reserved batch
     BatchKernelArgs ka;
     ka.descs = h->d_descs;
     ka.tile_lat = h->d_tile_lat;
@@ -306,114 +443,124 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
     return ISING_OK;
 }
 
-extern "C" {
+static int batch_check_modes(int msc_mode, int msc_cap)
+{
+    if (msc_mode != 0 && msc_mode != 1) return ising_fail(ISING_EINVAL, "batch: msc_mode must be 0 or 1");
+    if (msc_cap < 1 || msc_cap > 15) return ising_fail(ISING_EINVAL, "batch: msc_cap in 1..15");
+    return ISING_OK;
+}
 
-typedef struct {
-    int kind;       // 0 = int8, 1 = msc
-    int ndim;       // 2 or 3 (msc: 2)
-    int heatbath;   // int8 only
-    int lz, ly, lx; // lattice shape, lz = 1 in 2D
-    unsigned long long seed;
-    uint32_t table[7];  // thresholds, 2 * ndim + 1 used (msc: table[3], table[4])
-    void* c0;       // device colour arrays, caller-owned
-    void* c1;
-    uint32_t t0;    // sweep counter of the lattice when the batch starts
-} IsingBatchLattice;
+extern "C" {
 
 int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, int msc_mode,
                        int msc_cap, void** out)
 {
     if (!lats || n <= 0 || !out) return ising_fail(ISING_EINVAL, "batch_create: bad argument");
-    if (msc_mode != 0 && msc_mode != 1) return ising_fail(ISING_EINVAL, "batch_create: msc_mode must be 0 or 1");
-    if (msc_cap < 1 || msc_cap > 15) return ising_fail(ISING_EINVAL, "batch_create: msc_cap in 1..15");
+    if (int rc = batch_check_modes(msc_mode, msc_cap)) return rc;
     *out = nullptr;
-    std::vector<BatchDesc> descs((size_t)n);
+    std::vector<BatchDesc> descs;
     std::vector<u32> tile_lat;
-    u32 tiles = 0;
-    for (int i = 0; i < n; ++i) {
-        const IsingBatchLattice& L = lats[i];
-        BatchDesc& d = descs[(size_t)i];
-        d = BatchDesc{};
-        if (!L.c0 || !L.c1) return ising_fail(ISING_EINVAL, "batch_create: null colour array");
-        if (L.lz < 1 || L.ly < 2 || L.lx < 8 || (L.ly & 1) || (L.lx & 1) || (L.ndim == 3 && (L.lz < 2 || (L.lz & 1))))
-            return ising_fail(ISING_EINVAL, "batch_create: bad shape");
-        d.c0 = L.c0;
-        d.c1 = L.c1;
-        d.kind = (u32)L.kind;
-        d.ndim = (u32)L.ndim;
-        d.hb = (u32)(L.heatbath != 0);
-        d.k0 = (u32)(L.seed & 0xFFFFFFFFull);
-        d.k1 = (u32)(L.seed >> 32);
-        d.t0 = L.t0;
-        d.n_spins = (long long)L.lz * L.ly * L.lx;
-        const long long rows = (long long)L.lz * L.ly;
-        if (L.kind == BATCH_INT8) {
-            if (L.ndim != 2 && L.ndim != 3) return ising_fail(ISING_EINVAL, "batch_create: int8 ndim 2 or 3");
-            if (L.ndim == 2 && L.lz != 1) return ising_fail(ISING_EINVAL, "batch_create: 2D needs lz = 1");
-            const u32 lh = (u32)(L.lx / 2);
-            const u32 v = (lh % 16 == 0) ? 16u : (lh % 8 == 0) ? 8u : (lh % 4 == 0) ? 4u : 0u;
-            if (!v) return ising_fail(ISING_EINVAL, "batch_create: int8 needs Lx % 8 == 0");
-            d.vec = v;
-            d.g.lh = lh;
-            d.g.ly = (u32)L.ly;
-            d.g.lz = (u32)L.lz;
-            d.g.chunks_row = lh / v;
-            d.g.n_chunks = (u32)(rows * (lh / v));
-            d.g.div_chunks = make_fastdiv(d.g.chunks_row);
-            d.g.div_ly = make_fastdiv(d.g.ly);
-            for (int k = 0; k <= 2 * L.ndim; ++k) d.tb.v[k] = L.table[k];
-            d.n_tiles = (d.g.n_chunks + 255u) / 256u;
-        } else if (L.kind == BATCH_MSC) {
-            if (L.ndim != 2 || L.lz != 1 || L.lx % 128) return ising_fail(ISING_EINVAL, "batch_create: msc needs 2D, Lx % 128 == 0");
-            d.rows = (u32)L.ly;
-            d.nw = (u32)(L.lx / 128);
-            d.n_words = d.rows * d.nw;
-            d.div_nw = make_fastdiv(d.nw);
-            d.codes = msc_level_codes(L.table[3], L.table[4]);
-            const u32 per = msc_mode == 0 ? 512u : 256u;
-            d.n_tiles = (d.n_words + per - 1u) / per;
-        } else {
-            return ising_fail(ISING_EINVAL, "batch_create: kind must be 0 (int8) or 1 (msc)");
-        }
-        d.tile_begin = tiles;
-        tiles += d.n_tiles;
-        for (u32 k = 0; k < d.n_tiles; ++k) tile_lat.push_back((u32)i);
-    }
-    int dev = 0, n_cu = 0, max_per_cu = 0;
-    ISING_HIP(hipGetDevice(&dev));
-    ISING_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    ISING_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&max_per_cu, (const void*)batch_persistent, 256, 0));
-    if (max_per_cu < 1) return ising_fail(ISING_EHIP, "batch_create: kernel does not fit on a CU");
-    const int per_cu = (blocks_per_cu <= 0 || blocks_per_cu > max_per_cu) ? max_per_cu : blocks_per_cu;
+    if (int rc = batch_build(lats, n, (u32)msc_mode, descs, tile_lat)) return rc;
+    const u32 tiles = (u32)tile_lat.size();
 
     BatchHandle* h = new BatchHandle();
-    h->n_lat = (u32)n;
-    h->total_tiles = tiles;
-    h->grid = per_cu * n_cu;
-    if ((u32)h->grid > tiles) h->grid = (int)tiles;
-    h->max_blocks = max_per_cu;
     h->msc_mode = (u32)msc_mode;
     h->msc_cap = (u32)msc_cap;
-    hipError_t e = hipMalloc((void**)&h->d_descs, sizeof(BatchDesc) * (size_t)n);
-    if (e == hipSuccess) e = hipMalloc((void**)&h->d_tile_lat, sizeof(u32) * tile_lat.size());
-    if (e == hipSuccess) e = hipMalloc((void**)&h->d_plus, sizeof(u32) * 4 * (size_t)tiles);
-    if (e == hipSuccess) e = hipMalloc((void**)&h->d_acc, sizeof(BatchAcc) * (size_t)n);
-    if (e == hipSuccess) e = hipMalloc((void**)&h->d_done, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemcpy(h->d_descs, descs.data(), sizeof(BatchDesc) * (size_t)n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(h->d_tile_lat, tile_lat.data(), sizeof(u32) * tile_lat.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(256), 0, 0, (unsigned long long*)h->d_acc,
-                           (u32)(sizeof(BatchAcc) / 8) * (u32)n);
-        hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(64), 0, 0, h->d_done, 1u);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
+    int rc = batch_alloc(h, (u32)n, tiles, blocks_per_cu);
+    if (rc) {
         batch_free(h);
-        return ising_check_hip(e, "batch_create");
+        return rc;
+    }
+    batch_set_sizes(h, (u32)n, tiles);
+    hipError_t e = hipMemcpy(h->d_descs, descs.data(), sizeof(BatchDesc) * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_tile_lat, tile_lat.data(), sizeof(u32) * tile_lat.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) rc = batch_zero_stats(h, 0);
+    if (e == hipSuccess && !rc) e = hipDeviceSynchronize();
+    if (e != hipSuccess || rc) {
+        batch_free(h);
+        return rc ? rc : ising_check_hip(e, "batch_create");
     }
     *out = h;
     return ISING_OK;
+}
+
+// Number of tiles one half sweep of these lattices has (host only, no device call): the
+// capacity a reserved batch needs for them.
+int ising_batch_tiles(const IsingBatchLattice* lats, int n, int msc_mode, long long* tiles)
+{
+    if ((!lats && n) || n < 0 || !tiles) return ising_fail(ISING_EINVAL, "batch_tiles: bad argument");
+    if (int rc = batch_check_modes(msc_mode, 4)) return rc;
+    std::vector<BatchDesc> descs;
+    std::vector<u32> tile_lat;
+    if (int rc = batch_build(lats, n, (u32)msc_mode, descs, tile_lat)) return rc;
+    *tiles = (long long)tile_lat.size();
+    return ISING_OK;
+}
+
+// A batch with room for max_lattices lattices and max_tiles tiles and no lattices yet; fill it
+// with ising_batch_assign. Allocates (synchronously) once; later assigns allocate nothing.
+int ising_batch_reserve(int max_lattices, long long max_tiles, int blocks_per_cu, int msc_mode,
+                        int msc_cap, void** out)
+{
+    if (!out || max_lattices <= 0 || max_tiles <= 0 || max_tiles > 0x7FFFFFFFll)
+        return ising_fail(ISING_EINVAL, "batch_reserve: bad argument");
+    if (int rc = batch_check_modes(msc_mode, msc_cap)) return rc;
+    *out = nullptr;
+    BatchHandle* h = new BatchHandle();
+    h->msc_mode = (u32)msc_mode;
+    h->msc_cap = (u32)msc_cap;
+    int rc = batch_alloc(h, (u32)max_lattices, (u32)max_tiles, blocks_per_cu);
+    const size_t bytes = sizeof(BatchDesc) * (size_t)max_lattices + sizeof(u32) * (size_t)max_tiles;
+    for (int s = 0; s < 2 && !rc; ++s) {
+        hipError_t e = hipHostMalloc(&h->pinned[s], bytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->pinned_done[s], hipEventDisableTiming);
+        if (e != hipSuccess) rc = ising_check_hip(e, "batch_reserve");
+    }
+    batch_set_sizes(h, 0, 0);
+    if (!rc) rc = batch_zero_stats(h, 0);
+    if (!rc) rc = ising_check_hip(hipDeviceSynchronize(), "batch_reserve");
+    if (rc) {
+        batch_free(h);
+        return rc;
+    }
+    *out = h;
+    return ISING_OK;
+}
+
+// Replace the lattices of a reserved batch (n may be 0). Stream-ordered on `stream`: launches
+// queued there before this call still see the old list, launches after it the new one; the
+// per-lattice accumulators and the batch sweep counter restart at zero (lattice i continues
+// from its own t0). Allocates nothing; destroys a captured graph. ISING_ERANGE when the
+// lattices exceed the reserved capacity.
+int ising_batch_assign(void* handle, const IsingBatchLattice* lats, int n, void* stream)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || n < 0 || (n && !lats)) return ising_fail(ISING_EINVAL, "batch_assign: bad argument");
+    if (!h->pinned[0]) return ising_fail(ISING_EINVAL, "batch_assign: handle was not reserved");
+    if ((u32)n > h->cap_lat) return ising_fail(ISING_ERANGE, "batch_assign: more lattices than reserved");
+    std::vector<BatchDesc> descs;
+    std::vector<u32> tile_lat;
+    if (int rc = batch_build(lats, n, h->msc_mode, descs, tile_lat)) return rc;
+    if (tile_lat.size() > h->cap_tiles) return ising_fail(ISING_ERANGE, "batch_assign: more tiles than reserved");
+    hipStream_t st = as_stream(stream);
+    const int s = h->next_slot;
+    h->next_slot ^= 1;
+    ISING_HIP(hipEventSynchronize(h->pinned_done[s]));  // this slot's previous upload is done
+    char* stage = (char*)h->pinned[s];
+    const size_t desc_bytes = sizeof(BatchDesc) * (size_t)n;
+    const size_t map_bytes = sizeof(u32) * tile_lat.size();
+    if (n) {
+        memcpy(stage, descs.data(), desc_bytes);
+        ISING_HIP(hipMemcpyAsync(h->d_descs, stage, desc_bytes, hipMemcpyHostToDevice, st));
+    }
+    if (map_bytes) {
+        memcpy(stage + desc_bytes, tile_lat.data(), map_bytes);
+        ISING_HIP(hipMemcpyAsync(h->d_tile_lat, stage + desc_bytes, map_bytes, hipMemcpyHostToDevice, st));
+    }
+    ISING_HIP(hipEventRecord(h->pinned_done[s], st));
+    batch_drop_graph(h);
+    batch_set_sizes(h, (u32)n, (u32)tile_lat.size());
+    return batch_zero_stats(h, st);
 }
 
 // n_sweeps sweeps of every lattice, one cooperative launch (stream-ordered, asynchronous).
@@ -435,14 +582,8 @@ int ising_batch_graph(void* handle, long long sweeps_per_replay, void* stream)
         return ising_fail(ISING_EINVAL, "batch_graph: bad argument");
     hipStream_t st = as_stream(stream);
     if (!st) return ising_fail(ISING_EINVAL, "batch_graph: capture needs a created stream");
-    if (h->graph_exec) {
-        (void)hipGraphExecDestroy(h->graph_exec);
-        h->graph_exec = nullptr;
-    }
-    if (h->graph) {
-        (void)hipGraphDestroy(h->graph);
-        h->graph = nullptr;
-    }
+    if (!h->total_tiles) return ising_fail(ISING_EINVAL, "batch_graph: the batch has no lattices");
+    batch_drop_graph(h);
     ISING_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = batch_launch(h, (u32)sweeps_per_replay, st);
     hipGraph_t g = nullptr;
@@ -486,6 +627,7 @@ int ising_batch_reset_stats(void* handle, void* stream)
 {
     BatchHandle* h = (BatchHandle*)handle;
     if (!h) return ising_fail(ISING_EINVAL, "batch_reset_stats: bad handle");
+    if (!h->n_lat) return ISING_OK;
     hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(256), 0, as_stream(stream),
                        (unsigned long long*)h->d_acc, (u32)(sizeof(BatchAcc) / 8) * h->n_lat);
     ISING_LAUNCHED("batch_zero_u64");
@@ -500,15 +642,7 @@ int ising_batch_set_persistent(void* handle, int persistent)
     BatchHandle* h = (BatchHandle*)handle;
     if (!h || (persistent != 0 && persistent != 1))
         return ising_fail(ISING_EINVAL, "batch_set_persistent: bad argument");
-    if (h->graph_exec) {
-        (void)hipGraphExecDestroy(h->graph_exec);
-        h->graph_exec = nullptr;
-    }
-    if (h->graph) {
-        (void)hipGraphDestroy(h->graph);
-        h->graph = nullptr;
-    }
-    h->graph_sweeps = 0;
+    batch_drop_graph(h);
     h->persistent = persistent;
     return ISING_OK;
 }

@@ -18,6 +18,7 @@ int ising_fail(int code, const std::string& msg)
 
 int ising_check_hip(hipError_t e, const char* what)
 {
+    if (e == hipSuccess) return ISING_OK;
     g_last_error = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
     return ISING_EHIP;
 }
@@ -187,6 +188,19 @@ int ising_event_elapsed_ms(float* ms, void* start, void* stop)
 {
     ISING_HIP(hipEventSynchronize((hipEvent_t)stop));
     ISING_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return ISING_OK;
+}
+
+int ising_event_sync(void* ev)
+{
+    ISING_HIP(hipEventSynchronize((hipEvent_t)ev));
+    return ISING_OK;
+}
+
+// Work queued on `stream` after this call waits for the last record of `ev`.
+int ising_stream_wait_event(void* stream, void* ev)
+{
+    ISING_HIP(hipStreamWaitEvent(as_stream(stream), (hipEvent_t)ev, 0));
     return ISING_OK;
 }
 
