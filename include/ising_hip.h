@@ -109,10 +109,11 @@ int ising_int8_obs(const int8_t* d_c0, const int8_t* d_c1, long long rows, int l
                    int ndim, unsigned long long* d_scratch2, long long* d_out2, void* stream);
 
 /* ------------------------------------------------------ multi-spin coding (2D) ---- */
-/* Process default msc kernel: mode 0 lockstep + survivor compaction (k = 10 * cap + R, cap in
- * 1..15 lockstep levels, R in {1,2,4} x 256 words per workgroup; default k = 42), 1 lockstep,
- * 2 eager (all 16 calls; the msc-eager32 ablation of cupy_backend.py:176-180). All modes give
- * identical words. No reference counterpart (kernel selection). */
+/* Process default msc kernel: mode 0 queue compaction (k = 10 * cap + R: cap in 3..6 unrolled
+ * Philox calls per word, then packed one-call rounds for the undecided words; R in {2,4} x 256
+ * words per workgroup; default k = 42), 1 lockstep, 2 eager (all 16 calls; the msc-eager32
+ * ablation of cupy_backend.py:176-180). All modes give identical words. No reference
+ * counterpart (kernel selection). */
 int ising_msc_config(int mode, int k);
 /* One half sweep with the process default kernel. thr4 = table[3], thr8 = table[4].
  * stats (optional device uint64[3], accumulated): Philox calls, lane-slots executed, words.
@@ -157,8 +158,8 @@ typedef struct {
     void* c1;
     uint32_t t0;         /* sweep counter at batch start */
 } IsingBatchLattice;
-/* blocks_per_cu <= 0: occupancy maximum; msc_mode 0 = compaction tiles (512 words),
- * 1 = lockstep tiles (256 words); msc_cap = lockstep levels before compaction (1..15). */
+/* blocks_per_cu <= 0: occupancy maximum; msc_mode 0 = queue-compaction tiles (512 words),
+ * 1 = lockstep tiles (256 words); msc_cap = unrolled calls of the queue tiles (must be 4). */
 int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, int msc_mode,
                        int msc_cap, void** handle);
 /* n_sweeps sweeps of every lattice in one cooperative launch (asynchronous). */

@@ -90,7 +90,7 @@ class BatchScan:
     """Sweep a list of hip simulations together with one persistent kernel per call.
 
     ``blocks_per_cu``: persistent grid size per CU (None = the occupancy maximum).
-    ``msc_mode``: 0 = lockstep + survivor compaction (512-word tiles), 1 = lockstep tiles.
+    ``msc_mode``: 0 = queue compaction (512-word tiles), 1 = lockstep tiles.
     ``persistent``: True = one cooperative launch per ``sweep`` call with a grid barrier between
     half sweeps; False (default) = one launch per half sweep over all lattices plus one
     accumulation launch per sweep (kernel boundaries are cheaper than a grid barrier here).

@@ -3,8 +3,8 @@
 // reference runs one lattice per launch sequence, src/pyising/backends/base.py:85-90).
 //
 // Work decomposition. Every lattice is cut into tiles of one workgroup's work for one half
-// sweep: int8 = 256 threads x V sites, msc = 512 words (lockstep + survivor compaction) or 256
-// words (lockstep). A tile map lists (tile -> lattice) for all lattices; the grid strides over
+// sweep: int8 = 256 threads x V sites, msc = 512 words (queue compaction) or 256 words
+// (lockstep). A tile map lists (tile -> lattice) for all lattices; the grid strides over
 // it. Colour c + 1 of a lattice may only start when every tile of colour c of that lattice is
 // done: the kernel is launched cooperatively (all workgroups co-resident, checked by the
 // runtime) and a grid barrier separates the two half sweeps of every sweep, which orders all
@@ -103,6 +103,17 @@ __device__ __forceinline__ void batch_accumulate_lattice(const BatchKernelArgs& 
     }
 }
 
+// One 512-word msc tile by queue compaction (4 unrolled calls, then packed rounds); the
+// one-row-per-wave classification when the lattice's Lx % 8192 == 0 (block-uniform test).
+__device__ __forceinline__ u32 batch_msc_tile(const MscArgs& a, u64 codes, u32 base,
+                                              MscQueueLds<2>& lds)
+{
+    u32 calls = 0, slots = 0, words = 0;
+    if (a.nw % 64u == 0u)
+        return msc_queue_tile<2, 4, true, false, true, true>(a, codes, base, lds, calls, slots, words);
+    return msc_queue_tile<2, 4, false, false, true, true>(a, codes, base, lds, calls, slots, words);
+}
+
 template <int NDIM, int HB, int V>
 __device__ __forceinline__ u32 batch_int8_tile(const BatchDesc& d, u32 local, u32 t, u32 color)
 {
@@ -125,7 +136,7 @@ __device__ __forceinline__ u32 batch_int8_v(const BatchDesc& d, u32 local, u32 t
 
 __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
 {
-    __shared__ MscCompactLds<2> lds;
+    __shared__ MscQueueLds<2> lds;
     cg::grid_group grid = cg::this_grid();
     const u32 done0 = (u32)*ka.sweeps_done;
     const u32 gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -157,10 +168,9 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
                     a.color = color;
                     a.k0 = d.k0;
                     a.k1 = d.k1;
-                    u32 calls = 0, slots = 0, words = 0;
+                    u32 calls = 0, slots = 0;
                     if (ka.msc_mode == 0) {
-                        plus = msc_compact_tile<2, false, true>(a, d.codes, ka.msc_cap, local * 512u,
-                                                                lds, calls, slots, words);
+                        plus = batch_msc_tile(a, d.codes, local * 512u, lds);
                     } else {
                         const u32 w = local * 256u + threadIdx.x;
                         MscState st = {};
@@ -190,7 +200,7 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
 // the barrier between colours. Sweep index = sweeps_done (device) so captured graphs replay.
 __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
 {
-    __shared__ MscCompactLds<2> lds;
+    __shared__ MscQueueLds<2> lds;
     const u32 tile = xcd_band_block(blockIdx.x, gridDim.x);
     const u32 sw = (u32)*ka.sweeps_done;
     const u32 par = sw & 1u;
@@ -216,10 +226,9 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
         a.color = color;
         a.k0 = d.k0;
         a.k1 = d.k1;
-        u32 calls = 0, slots = 0, words = 0;
+        u32 calls = 0, slots = 0;
         if (ka.msc_mode == 0) {
-            plus = msc_compact_tile<2, false, true>(a, d.codes, ka.msc_cap, local * 512u, lds, calls,
-                                                    slots, words);
+            plus = batch_msc_tile(a, d.codes, local * 512u, lds);
         } else {
             const u32 w = local * 256u + threadIdx.x;
             MscState st = {};
@@ -446,7 +455,7 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
 static int batch_check_modes(int msc_mode, int msc_cap)
 {
     if (msc_mode != 0 && msc_mode != 1) return ising_fail(ISING_EINVAL, "batch: msc_mode must be 0 or 1");
-    if (msc_cap < 1 || msc_cap > 15) return ising_fail(ISING_EINVAL, "batch: msc_cap in 1..15");
+    if (msc_cap != 4) return ising_fail(ISING_EINVAL, "batch: msc_cap must be 4 (unrolled calls of the queue tiles)");
     return ISING_OK;
 }
 

@@ -90,6 +90,62 @@ __device__ __forceinline__ PhiloxOut philox4x32_10(u32 c0, u32 c1, u32 c2, u32 c
     return PhiloxOut{c0, c1, c2, c3};
 }
 
+// Wave-uniform value forced into a scalar register.
+__device__ __forceinline__ u32 uni(u32 x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Philox4x32-10 of the counter (c0, c1, c2, c3) where only c0 varies across the wave and c1,
+// c2, c3, k0, k1 are wave-uniform (REQUIRED: callers pass sweep, colour, level and key). The
+// caller passes (hi0, lo0) = c0 * M0, which is loop-invariant over levels. The uniform halves
+// of rounds 0-2 stay on the scalar unit and every per-lane XOR with two uniform inputs becomes
+// one VALU XOR: 36 VALU ops per call. Same outputs as philox4x32_10.
+__device__ __forceinline__ PhiloxOut philox_lane(u32 hi0, u32 lo0, u32 c1, u32 c2, u32 c3, u32 k0,
+                                                 u32 k1)
+{
+    // round 0: the c2 product is uniform
+    const u64 q1 = (u64)c2 * PHILOX_M1;
+    const u32 u0 = uni((u32)(q1 >> 32) ^ c1 ^ k0);
+    const u32 u1 = (u32)q1;
+    const u32 x2 = hi0 ^ uni(c3 ^ k1);
+    const u32 x3 = lo0;
+    k0 += PHILOX_W0;
+    k1 += PHILOX_W1;
+    // round 1: the first product is uniform
+    const u64 q0 = (u64)u0 * PHILOX_M0;
+    const u64 p1 = (u64)x2 * PHILOX_M1;
+    const u32 y0 = (u32)(p1 >> 32) ^ uni(u1 ^ k0);
+    const u32 y1 = (u32)p1;
+    const u32 y2 = x3 ^ uni((u32)(q0 >> 32) ^ k1);
+    const u32 y3 = uni((u32)q0);
+    k0 += PHILOX_W0;
+    k1 += PHILOX_W1;
+    // round 2: counter word 3 is still uniform
+    u32 c0;
+    {
+        const u64 p0 = (u64)y0 * PHILOX_M0;
+        const u64 pp = (u64)y2 * PHILOX_M1;
+        c0 = xor3((u32)(pp >> 32), y1, k0);
+        c1 = (u32)pp;
+        c2 = (u32)(p0 >> 32) ^ uni(y3 ^ k1);
+        c3 = (u32)p0;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+#pragma unroll
+    for (int r = 3; r < 10; ++r) {
+        const u64 p0 = (u64)c0 * PHILOX_M0;
+        const u64 p1r = (u64)c2 * PHILOX_M1;
+        const u32 n0 = xor3((u32)(p1r >> 32), c1, k0);
+        const u32 n2 = xor3((u32)(p0 >> 32), c3, k1);
+        c0 = n0;
+        c1 = (u32)p1r;
+        c2 = n2;
+        c3 = (u32)p0;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+    return PhiloxOut{c0, c1, c2, c3};
+}
+
 // ---------------------------------------------------------- division by a runtime constant ----
 // q = n / d for 32-bit n with a host-precomputed magic number (Granlund & Montgomery 1994,
 // round-up variant): t = mulhi(n, m); q = (t + ((n - t) >> s1)) >> s2. d == 1 uses m = 0,

@@ -22,40 +22,7 @@ struct MscState {
     u32 a1l, a1h, pwl, pwh, ul, uh;
 };
 
-// Neighbour classification of word w: five loads through the cache hierarchy and bit-sliced
-// counting of anti-aligned neighbours.
-__device__ __forceinline__ MscState msc_classify(const MscArgs& a, u32 w)
-{
-    const u32 y = fast_div(w, a.div_nw);
-    const u32 x = w - y * a.nw;
-    const u64 rbase = (u64)y * a.nw;
-    const u32 p = (y + a.color) & 1u;
-    const u64 o = a.own[w];
-    const u64 c = a.other[w];
-    const u32 xn = p ? ((x + 1 == a.nw) ? 0u : x + 1) : ((x == 0) ? a.nw - 1 : x - 1);
-    const u64 nb = a.other[rbase + xn];
-    const u64 hz = p ? ((c >> 1) | (nb << 63)) : ((c << 1) | (nb >> 63));
-    const u32 yu = (y + 1 == a.rows) ? 0u : y + 1;
-    const u32 yd = (y == 0) ? a.rows - 1 : y - 1;
-    const u64 up = a.other[(u64)yu * a.nw + x];
-    const u64 dn = a.other[(u64)yd * a.nw + x];
-    const u64 x1 = o ^ c, x2 = o ^ hz, x3 = o ^ up, x4 = o ^ dn;
-    const u64 anyx = x1 | x2 | x3 | x4;
-    const u64 a2 = (x1 & x2) | (x3 & x4) | ((x1 ^ x2) & (x3 ^ x4));
-    const u64 a1 = anyx & ~a2;
-    const u64 und = ~a2;  // a0 | a1
-    const u64 pw = o ^ a2;
-    MscState s;
-    s.a1l = (u32)a1;
-    s.a1h = (u32)(a1 >> 32);
-    s.pwl = (u32)pw;
-    s.pwh = (u32)(pw >> 32);
-    s.ul = (u32)und;
-    s.uh = (u32)(und >> 32);
-    return s;
-}
-
-// ------------------------------------------------------------------ lazy comparison ----
+// ------------------------------------------------------------------ classification ----
 // Three-input boolean functions as single v_bitop3_b32 ops; the immediate is the function
 // evaluated on the operand patterns (0xF0, 0xCC, 0xAA).
 template <typename F>
@@ -63,6 +30,147 @@ constexpr u32 b3imm(F f)
 {
     return f(0xF0u, 0xCCu, 0xAAu) & 0xFFu;
 }
+constexpr u32 B3_MAJ = b3imm([](u32 x, u32 y, u32 z) { return (x & y) | (x & z) | (y & z); });
+constexpr u32 B3_OR3 = b3imm([](u32 x, u32 y, u32 z) { return x | y | z; });
+constexpr u32 B3_NOT_A_OR_BC = b3imm([](u32 x, u32 y, u32 z) { return ~(x | (y & z)); });
+constexpr u32 B3_AB_OR_AND_C = b3imm([](u32 x, u32 y, u32 z) { return (x | y) & z; });
+static_assert(B3_MAJ == 0xE8u && B3_OR3 == 0xFEu && B3_NOT_A_OR_BC == 0x07u && B3_AB_OR_AND_C == 0xA8u,
+              "bitop3 truth tables");
+
+#define bop3(a, b, c, imm) ((u32)__builtin_amdgcn_bitop3_b32((a), (b), (c), (imm)))
+
+// One 32-bit half: own spins o, other-colour neighbours (c, hz, up, dn). With x_k = o ^ n_k
+// (anti-aligned), a2 = at least two of x1..x4 = maj(x1, x2, x3) | (x4 & (x1 | x2 | x3)),
+// und = ~a2 (lanes that need a comparison), a1 = exactly one = (x1 | x2 | x3 | x4) & und,
+// pw = o ^ a2 (a2 lanes flip unconditionally). Nine VALU ops.
+__device__ __forceinline__ void msc_bits(u32 o, u32 c, u32 hz, u32 up, u32 dn, u32& a1, u32& pw,
+                                         u32& und)
+{
+    const u32 x1 = o ^ c, x2 = o ^ hz, x3 = o ^ up, x4 = o ^ dn;
+    const u32 m = bop3(x1, x2, x3, B3_MAJ);
+    const u32 r = bop3(x1, x2, x3, B3_OR3);
+    und = bop3(m, x4, r, B3_NOT_A_OR_BC);
+    a1 = bop3(r, x4, und, B3_AB_OR_AND_C);
+    pw = ~(o ^ und);
+}
+
+__device__ __forceinline__ MscState msc_state(u64 o, u64 c, u64 hz, u64 up, u64 dn)
+{
+    MscState s;
+    msc_bits((u32)o, (u32)c, (u32)hz, (u32)up, (u32)dn, s.a1l, s.pwl, s.ul);
+    msc_bits((u32)(o >> 32), (u32)(c >> 32), (u32)(hz >> 32), (u32)(up >> 32), (u32)(dn >> 32),
+             s.a1h, s.pwh, s.uh);
+    return s;
+}
+
+// Classification of the 64 consecutive words w0 .. w0 + 63 of one row by the calling wave
+// (requires nw % 64 == 0 and w0 % 64 == 0; w0 wave-uniform). Row, parity and the row pointers
+// are scalar; the four same-column loads use one scalar base and the lane offset. The
+// horizontal neighbour's boundary bit comes from the adjacent lane's word through a DPP
+// wavefront shift; the wave's edge lane takes it from one uniform load of the word beyond.
+// Split into the loads and the arithmetic so that callers can issue loads early.
+struct MscRowLoad {
+    u64 o, c, up, dn;
+    u32 e;      // boundary half-word of the word beyond the wave's edge (uniform)
+    u32 right;  // 1: colour sites at odd columns, horizontal neighbour j + 1 (uniform)
+};
+
+__device__ __forceinline__ MscRowLoad msc_load_row(const MscArgs& a, u32 w0, u32 lane)
+{
+    w0 = uni(w0);
+    const u32 y = fast_div(w0, a.div_nw);
+    const u32 x0 = w0 - y * a.nw;
+    const u32 yu = (y + 1u == a.rows) ? 0u : y + 1u;
+    const u32 yd = (y == 0u) ? a.rows - 1u : y - 1u;
+    const u64* crow = a.other + (u64)y * a.nw;
+    MscRowLoad L;
+    L.o = (a.own + w0)[lane];
+    L.c = (crow + x0)[lane];
+    L.up = (a.other + (u64)yu * a.nw + x0)[lane];
+    L.dn = (a.other + (u64)yd * a.nw + x0)[lane];
+    L.right = (y + a.color) & 1u;
+    if (L.right) {
+        const u32 xe = (x0 + 64u == a.nw) ? 0u : x0 + 64u;  // word x0 + 64, wrapped
+        L.e = (u32)crow[xe];
+    } else {
+        const u32 xe = (x0 == 0u) ? a.nw - 1u : x0 - 1u;  // word x0 - 1, wrapped
+        L.e = (u32)(crow[xe] >> 32);
+    }
+    return L;
+}
+
+__device__ __forceinline__ MscState msc_classify_row(const MscRowLoad& L)
+{
+    const u32 cl = (u32)L.c, ch = (u32)(L.c >> 32);
+    u32 hzl, hzh;
+    if (L.right) {
+        // hz = (c >> 1) | (next word << 63); wave_shl:1 -- lane i reads lane i + 1, lane 63
+        // has no source and keeps e
+        const u32 nb = (u32)__builtin_amdgcn_update_dpp((int)L.e, (int)cl, 0x130, 0xF, 0xF, false);
+        hzl = __builtin_amdgcn_alignbit(ch, cl, 1u);
+        hzh = __builtin_amdgcn_alignbit(nb, ch, 1u);
+    } else {
+        // hz = (c << 1) | (previous word >> 63); wave_shr:1 -- lane i reads lane i - 1, lane 0
+        // has no source and keeps e
+        const u32 nb = (u32)__builtin_amdgcn_update_dpp((int)L.e, (int)ch, 0x138, 0xF, 0xF, false);
+        hzh = __builtin_amdgcn_alignbit(ch, cl, 31u);
+        hzl = __builtin_amdgcn_alignbit(cl, nb, 31u);
+    }
+    return msc_state(L.o, L.c, (u64)hzl | ((u64)hzh << 32), L.up, L.dn);
+}
+
+// Neighbour loads of word w (any geometry): five loads through the cache hierarchy, the
+// horizontal neighbour already shifted into place.
+struct MscLoad {
+    u64 o, c, hz, up, dn;
+};
+
+__device__ __forceinline__ MscLoad msc_load(const MscArgs& a, u32 w)
+{
+    const u32 y = fast_div(w, a.div_nw);
+    const u32 x = w - y * a.nw;
+    const u64 rbase = (u64)y * a.nw;
+    const u32 p = (y + a.color) & 1u;
+    MscLoad L;
+    L.o = a.own[w];
+    L.c = a.other[w];
+    const u32 xn = p ? ((x + 1 == a.nw) ? 0u : x + 1) : ((x == 0) ? a.nw - 1 : x - 1);
+    const u64 nb = a.other[rbase + xn];
+    L.hz = p ? ((L.c >> 1) | (nb << 63)) : ((L.c << 1) | (nb >> 63));
+    const u32 yu = (y + 1 == a.rows) ? 0u : y + 1;
+    const u32 yd = (y == 0) ? a.rows - 1 : y - 1;
+    L.up = a.other[(u64)yu * a.nw + x];
+    L.dn = a.other[(u64)yd * a.nw + x];
+    return L;
+}
+
+__device__ __forceinline__ MscState msc_classify(const MscLoad& L)
+{
+    return msc_state(L.o, L.c, L.hz, L.up, L.dn);
+}
+
+__device__ __forceinline__ MscState msc_classify(const MscArgs& a, u32 w)
+{
+    return msc_classify(msc_load(a, w));
+}
+
+// Philox call of word w at level pair lp (lp wave-uniform): counter (w, t, colour, 1 + lp).
+struct MscCounter {
+    u32 hi0, lo0;  // w * M0, the per-lane half of Philox round 0
+};
+
+__device__ __forceinline__ MscCounter msc_counter(u32 w)
+{
+    const u64 p = (u64)w * PHILOX_M0;
+    return MscCounter{(u32)(p >> 32), (u32)p};
+}
+
+__device__ __forceinline__ PhiloxOut msc_philox(const MscArgs& a, const MscCounter& c, u32 lp)
+{
+    return philox_lane(c.hi0, c.lo0, a.t, a.color, 1u + lp, a.k0, a.k1);
+}
+
+// ------------------------------------------------------------------ lazy comparison ----
 constexpr u32 B3_A_ANDN_B = b3imm([](u32 x, u32 y, u32) { return x & ~y; });
 constexpr u32 B3_A_XOR_B_ANDN_C = b3imm([](u32 x, u32 y, u32 z) { return x ^ (y & ~z); });
 constexpr u32 B3_A_B_ANDN_C = b3imm([](u32 x, u32 y, u32 z) { return x & y & ~z; });
@@ -72,8 +180,6 @@ constexpr u32 B3_A_AND_B_XOR_C = b3imm([](u32 x, u32 y, u32 z) { return x & (y ^
 static_assert(B3_A_ANDN_B == 0x30u && B3_A_XOR_B_ANDN_C == 0xB4u && B3_A_B_ANDN_C == 0x40u &&
                   B3_A_XNOR_BC == 0x90u && B3_A_ANDN_B_ANDN_C == 0x10u && B3_A_AND_B_XOR_C == 0x60u,
               "bitop3 truth tables");
-
-#define bop3(a, b, c, imm) ((u32)__builtin_amdgcn_bitop3_b32((a), (b), (c), (imm)))
 
 // One level on a 32-bit half for threshold bits (B4 of thr4, B8 of thr8): a1 lanes compare
 // their uniform's bit with B4, a0 lanes with B8; a lane whose bit differs is decided (u bit 0
@@ -147,12 +253,13 @@ template <bool LAZY, bool STATS>
 __device__ __forceinline__ u32 msc_run(const MscArgs& a, u64 codes, u32 w, u32 lp, u32 lp_end,
                                        MscState& s, u32& calls, u32& slots)
 {
+    const MscCounter ctr = msc_counter(w);
     while (lp < lp_end && (!LAZY || __ballot((s.ul | s.uh) != 0u) != 0)) {
         if (STATS) {
             calls += (!LAZY || (s.ul | s.uh) != 0u) ? 1u : 0u;
             ++slots;
         }
-        const PhiloxOut r = philox4x32_10(w, a.t, a.color, 1u + lp, a.k0, a.k1);
+        const PhiloxOut r = msc_philox(a, ctr, lp);
         msc_pair_switch((u32)(codes >> (4u * lp)) & 15u, r, s);
         ++lp;
     }
@@ -176,89 +283,215 @@ __device__ __forceinline__ void msc_stats(unsigned long long* stats, u32 calls, 
 }
 
 
-// LDS of one compaction tile: survivor state and the survivor queue of W = 256 * R words.
+// Store of a finished word. NT: non-temporal (streaming) store, so the new spins leave the
+// L2 during the kernel instead of in the write-back at the kernel boundary.
+template <bool NT>
+__device__ __forceinline__ void msc_store(u64* p, const MscState& s)
+{
+    const u64 v = (u64)s.pwl | ((u64)s.pwh << 32);
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// LDS of one queue-compaction tile (W = 256 * R words): survivor state by slot, two survivor
+// queues used alternately, and the queue length after phase A and after every phase-B round.
 template <int R>
-struct MscCompactLds {
+struct MscQueueLds {
     u64 a1[256 * R], und[256 * R], pw[256 * R];
-    uint16_t q[256 * R];
-    u32 qn, next;
+    uint16_t q[2][256 * R];
+    u32 qn[18];
 };
 
-// Lockstep + survivor compaction over the W = 256 * R words starting at `base`, executed by
-// one 256-thread workgroup (all threads must call it; it contains block barriers).
-// Phase A: R rounds of one word per lane, lockstep levels 0 .. cap-1 (nearly every word needs
-// them at T_c); decided words are stored, the others' state (a1, und, pw) parked in LDS and
-// their slot appended to a queue (one LDS atomic per wave). One barrier.
-// Phase B: waves take 64 queued survivors at a time (LDS batch counter) and run them lockstep
-// from level cap to completion; all start at the same level, so the level index stays
-// uniform, and the wave pays only the maximum of the survivors' remaining calls.
+template <int R>
+__device__ __forceinline__ void msc_park(MscQueueLds<R>& L, u32 slot, const MscState& s)
+{
+    L.a1[slot] = (u64)s.a1l | ((u64)s.a1h << 32);
+    L.und[slot] = (u64)s.ul | ((u64)s.uh << 32);
+    L.pw[slot] = (u64)s.pwl | ((u64)s.pwh << 32);
+}
+
+template <int R>
+__device__ __forceinline__ MscState msc_unpark(const MscQueueLds<R>& L, u32 slot)
+{
+    const u64 a1 = L.a1[slot], und = L.und[slot], pw = L.pw[slot];
+    MscState s;
+    s.a1l = (u32)a1;
+    s.a1h = (u32)(a1 >> 32);
+    s.ul = (u32)und;
+    s.uh = (u32)(und >> 32);
+    s.pwl = (u32)pw;
+    s.pwh = (u32)(pw >> 32);
+    return s;
+}
+
+// Append the lanes of `keep` to queue `q` (one LDS atomic per wave) and park their state.
+template <int R>
+__device__ __forceinline__ void msc_enqueue(MscQueueLds<R>& L, uint16_t* q, u32* qn, bool keep,
+                                            u32 slot, const MscState& s, bool state_a1)
+{
+    const u64 km = __ballot(keep);
+    u32 qb = 0;
+    if ((threadIdx.x & 63u) == 0u && km) qb = atomicAdd(qn, (u32)__popcll(km));
+    qb = __builtin_amdgcn_readlane(qb, 0);
+    if (keep) {
+        if (state_a1) L.a1[slot] = (u64)s.a1l | ((u64)s.a1h << 32);
+        L.und[slot] = (u64)s.ul | ((u64)s.uh << 32);
+        L.pw[slot] = (u64)s.pwl | ((u64)s.pwh << 32);
+        q[qb + lane_rank(km)] = (uint16_t)slot;
+    }
+}
+
+// Queue compaction over the W = 256 * R words starting at `base`, one 256-thread workgroup
+// (all threads must call it; it contains block barriers).
+// Phase A: R rounds of one word per lane, CAP lockstep Philox calls unrolled without exit
+//   tests (at T_c nearly every word needs them); decided words are stored, survivors parked.
+// Phase B: rounds of exactly one call per survivor. Survivors are packed into full waves, all
+//   at the same level (so the level's case code stays wave-uniform); those still undecided
+//   are queued for the next round. A wave slot is wasted only in the last partial wave of a
+//   round, instead of on every lane that finished before the slowest of its 64 words.
+// Tail: once at most 64 survivors remain, one wave runs them lockstep to completion.
+// ROWW: nw % 64 == 0 (every wave's 64 words lie in one row, see msc_classify_row).
 // Returns, when COUNT, this thread's share of the number of +1 spins written.
-template <int R, bool STATS, bool COUNT>
-__device__ __forceinline__ u32 msc_compact_tile(const MscArgs& a, u64 codes, u32 cap, u32 base,
-                                                MscCompactLds<R>& L, u32& calls, u32& slots,
-                                                u32& words)
+// Phase A's loads of one tile, issued before any of its arithmetic (the words of every round
+// are independent), so that memory latency hides behind earlier work.
+template <int R, bool ROWW>
+struct MscTileLoads {
+    MscRowLoad lr[ROWW ? R : 1];
+    MscLoad lg[ROWW ? 1 : R];
+};
+
+// Branch-free: words past the end load a clamped (valid) address instead, and their results
+// are discarded by the caller.
+template <int R, bool ROWW>
+__device__ __forceinline__ void msc_tile_load(const MscArgs& a, u32 base, MscTileLoads<R, ROWW>& T)
 {
     const u32 tid = threadIdx.x;
-    const u32 lane = tid & 63u;
-    u32 ones = 0;
-    if (tid == 0) {
-        L.qn = 0;
-        L.next = 0;
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+        if (ROWW) {
+            // n_words is a multiple of 64: clamp the wave's first word
+            const u32 w0 = base + (u32)rr * 256u + (tid & ~63u);
+            T.lr[rr] = msc_load_row(a, w0 < a.n_words ? w0 : a.n_words - 64u, tid & 63u);
+        } else {
+            const u32 w = base + (u32)rr * 256u + tid;
+            T.lg[rr] = msc_load(a, w < a.n_words ? w : a.n_words - 1u);
+        }
     }
+}
+
+struct MscNoPrefetch {
+    __device__ void operator()() const {}
+};
+
+template <int R, int CAP, bool ROWW, bool STATS, bool COUNT, bool NT, class PF>
+__device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes, u32 base,
+                                                     MscQueueLds<R>& L,
+                                                     const MscTileLoads<R, ROWW>& T, PF prefetch,
+                                                     u32& calls, u32& slots, u32& words)
+{
+    static_assert(CAP >= 1 && CAP <= 15, "phase A levels");
+    const u32 tid = threadIdx.x;
+    const u32 lane = tid & 63u;
+    const u32 wave = tid >> 6;
+    u32 ones = 0;
+    if (tid < 18u) L.qn[tid] = 0u;
     __syncthreads();
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
         const u32 slot = (u32)rr * 256u + tid;
         const u32 w = base + slot;
         const bool valid = w < a.n_words;
-        MscState s = {};
-        if (valid) {
-            s = msc_classify(a, w);
-            ++words;
+        // the CAP Philox calls depend on the counter only: computed before the spin loads are
+        // consumed, they cover the memory latency (at the start of every workgroup above all)
+        const MscCounter ctr = msc_counter(w);
+        PhiloxOut r[CAP];
+#pragma unroll
+        for (int lp = 0; lp < CAP; ++lp) {
+            r[lp] = msc_philox(a, ctr, (u32)lp);
+            // an empty asm that "modifies" the outputs pins the call here: the optimiser would
+            // otherwise sink it below the classification and its wait for the loads
+            asm volatile("" : "+v"(r[lp].x), "+v"(r[lp].y), "+v"(r[lp].z), "+v"(r[lp].w));
         }
-        msc_run<true, STATS>(a, codes, w, 0u, cap, s, calls, slots);
-        const bool park = valid && (s.ul | s.uh) != 0u;
+        __builtin_amdgcn_sched_barrier(0);
+        MscState s = ROWW ? msc_classify_row(T.lr[ROWW ? rr : 0]) : msc_classify(T.lg[ROWW ? 0 : rr]);
+        if (!valid) {
+            s.ul = 0u;  // nothing to decide, store or park
+            s.uh = 0u;
+        }
+        if (valid) ++words;
+#pragma unroll
+        for (int lp = 0; lp < CAP; ++lp) {
+            if (STATS) {
+                calls += (s.ul | s.uh) != 0u ? 1u : 0u;
+                ++slots;
+            }
+            msc_pair_switch((u32)(codes >> (4u * (u32)lp)) & 15u, r[lp], s);
+        }
+        const bool park = (s.ul | s.uh) != 0u;  // invalid lanes have und = 0
         if (valid && !park) {
-            a.own[w] = (u64)s.pwl | ((u64)s.pwh << 32);
+            msc_store<NT>(a.own + w, s);
             if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
         }
-        const u64 pm = __ballot(park);
-        u32 qb = 0;
-        if (lane == 0 && pm) qb = atomicAdd(&L.qn, (u32)__popcll(pm));
-        qb = __shfl(qb, 0, 64);
-        if (park) {
-            L.a1[slot] = (u64)s.a1l | ((u64)s.a1h << 32);
-            L.und[slot] = (u64)s.ul | ((u64)s.uh << 32);
-            L.pw[slot] = (u64)s.pwl | ((u64)s.pwh << 32);
-            L.q[qb + lane_rank(pm)] = (uint16_t)slot;
-        }
+        msc_enqueue(L, L.q[0], &L.qn[0], park, slot, s, true);
     }
     __syncthreads();
-    const u32 n = L.qn;
-    while (true) {
-        u32 b = 0;
-        if (lane == 0) b = atomicAdd(&L.next, 64u);
-        b = __shfl(b, 0, 64);
-        if (b >= n) break;  // wave-uniform
-        const u32 i = b + lane;
+    prefetch();  // e.g. the next tile's loads, which then complete while phase B runs
+    u32 n = L.qn[0];
+    u32 cur = 0;
+    u32 lp = CAP;
+    while (n > 64u && lp < 16u) {  // block-uniform
+        const u32 code = (u32)(codes >> (4u * lp)) & 15u;
+        for (u32 i0 = wave * 64u; i0 < n; i0 += 256u) {  // wave-uniform
+            const u32 i = i0 + lane;
+            MscState s = {};
+            u32 slot = 0;
+            if (i < n) {
+                slot = L.q[cur][i];
+                s = msc_unpark(L, slot);
+            }
+            if (STATS) {
+                calls += (s.ul | s.uh) != 0u ? 1u : 0u;
+                ++slots;
+            }
+            const PhiloxOut r = msc_philox(a, msc_counter(base + slot), lp);
+            msc_pair_switch(code, r, s);
+            const bool still = (s.ul | s.uh) != 0u;
+            if (i < n && !still) {
+                msc_store<NT>(a.own + base + slot, s);
+                if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
+            }
+            msc_enqueue(L, L.q[cur ^ 1u], &L.qn[lp + 1u - CAP], still, slot, s, false);
+        }
+        __syncthreads();
+        n = L.qn[lp + 1u - CAP];
+        cur ^= 1u;
+        ++lp;
+    }
+    for (u32 i0 = wave * 64u; i0 < n; i0 += 256u) {  // n <= 64 unless every level is spent
+        const u32 i = i0 + lane;
         MscState s = {};
         u32 slot = 0;
         if (i < n) {
-            slot = L.q[i];
-            const u64 a1 = L.a1[slot], und = L.und[slot], pw = L.pw[slot];
-            s.a1l = (u32)a1;
-            s.a1h = (u32)(a1 >> 32);
-            s.ul = (u32)und;
-            s.uh = (u32)(und >> 32);
-            s.pwl = (u32)pw;
-            s.pwh = (u32)(pw >> 32);
+            slot = L.q[cur][i];
+            s = msc_unpark(L, slot);
         }
-        msc_run<true, STATS>(a, codes, base + slot, cap, 16u, s, calls, slots);
+        msc_run<true, STATS>(a, codes, base + slot, lp, 16u, s, calls, slots);
         if (i < n) {
-            a.own[base + slot] = (u64)s.pwl | ((u64)s.pwh << 32);
+            msc_store<NT>(a.own + base + slot, s);
             if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
         }
     }
     __syncthreads();  // the LDS of this tile may be reused by the caller's next tile
     return ones;
+}
+
+template <int R, int CAP, bool ROWW, bool STATS, bool COUNT, bool NT>
+__device__ __forceinline__ u32 msc_queue_tile(const MscArgs& a, u64 codes, u32 base,
+                                              MscQueueLds<R>& L, u32& calls, u32& slots,
+                                              u32& words)
+{
+    MscTileLoads<R, ROWW> T;
+    msc_tile_load<R, ROWW>(a, base, T);
+    return msc_queue_tile_loaded<R, CAP, ROWW, STATS, COUNT, NT>(a, codes, base, L, T,
+                                                             MscNoPrefetch{}, calls, slots, words);
 }

@@ -16,18 +16,24 @@
 // on average at T_c, up to 16). Any lane may evaluate any (word, level) call: the stream is
 // counter-based, which is what lets the kernels below regroup work.
 //
-// Cost model (measured with SQ counters on MI355X): Philox4x32-10 is 36 VALU ops, every op
-// including v_mad_u64_u32 at the full wave64 rate, so the kernel is VALU-issue bound and its
-// time is (lane-slots per word) x (ops per slot). Three things keep both factors low:
+// Cost model (measured with SQ counters on MI355X): Philox4x32-10 is 36 VALU ops (philox_lane:
+// the wave-uniform halves of rounds 0-2 run on the scalar unit), every op including
+// v_mad_u64_u32 at the full wave64 rate, so the kernel is VALU-issue bound and its time is
+// (lane-slots per word) x (ops per slot) plus a per-launch cost (dispatch, ramp, tail) of a few
+// microseconds. What keeps both factors low:
 //  * the level index lp is wave-uniform everywhere, so the threshold bits of a level are
 //    known to the whole wave: the host packs the 4-bit case code of every level pair into one
 //    64-bit argument and each call costs one scalar switch into straight-line v_bitop3 code
 //    (0-3 ops per 32-bit half and level) instead of a masked general update;
-//  * the loop is wave-uniform (no per-lane exit), so there is no exec-mask bookkeeping; a
+//  * loops are wave-uniform (no per-lane exit), so there is no exec-mask bookkeeping; a
 //    decided lane's updates are no-ops;
-//  * survivor compaction (mode 0): lockstep pays the slowest of 64 words; after `cap` levels
-//    only ~18% of words are undecided at T_c, so those are regrouped through LDS into full
-//    waves that start at the same level (still uniform) and run to completion.
+//  * classification is nine v_bitop3/XOR ops per 32-bit half; when a wave's 64 words lie in
+//    one row (Lx % 8192 == 0) its row pointers are scalar and the horizontal boundary bit
+//    moves between lanes by a DPP wavefront shift;
+//  * queue compaction (mode 0): `cap` Philox calls per word unrolled (computed before the spin
+//    loads are consumed, hiding their latency), then the ~15% of words still undecided run
+//    one call per round packed into full waves, so lockstep no longer pays the slowest of 64
+//    words; streaming stores keep the new spins out of the kernel-boundary L2 write-back.
 #include "msc_core.hpp"
 
 // ---------------------------------------------------------------------------- kernels ----
@@ -50,16 +56,17 @@ __global__ __launch_bounds__(256) void msc_half_lockstep(MscArgs a, u64 codes,
     if (STATS) msc_stats(stats, calls, slots, valid ? 1u : 0u);
 }
 
-// Lockstep + survivor compaction (the default): one workgroup per W = 256 * R words.
-// R = 2 uses 13 KB of LDS and 24 VGPRs: 8 workgroups (32 waves) per CU.
-template <int R, bool STATS>
-__global__ __launch_bounds__(256) void msc_half_compact(MscArgs a, u64 codes, u32 cap,
-                                                       unsigned long long* stats)
+// Queue compaction (the default): one workgroup per W = 256 * R words, CAP unrolled lockstep
+// calls, then rounds of one call per packed survivor (msc_queue_tile); streaming stores.
+// R = 2 uses 14 KB of LDS and 36 VGPRs: 8 workgroups (32 waves) per CU.
+template <int R, int CAP, bool ROWW, bool STATS>
+__global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u64 codes,
+                                                     unsigned long long* stats)
 {
-    __shared__ MscCompactLds<R> lds;
+    __shared__ MscQueueLds<R> lds;
     const u32 base = xcd_band_block(blockIdx.x, gridDim.x) * (256u * R);
     u32 calls = 0, slots = 0, words = 0;
-    msc_compact_tile<R, STATS, false>(a, codes, cap, base, lds, calls, slots, words);
+    msc_queue_tile<R, CAP, ROWW, STATS, false, true>(a, codes, base, lds, calls, slots, words);
     if (STATS) msc_stats(stats, calls, slots, words);
 }
 
@@ -116,9 +123,9 @@ __global__ void msc_zero2(unsigned long long* p)
 }
 
 // ---------------------------------------------------------------------- host launchers ----
-// Kernel modes: 0 = lockstep + survivor compaction (k = 10 * cap + R, cap in 1..15,
-// R in {1, 2, 4}; default cap 4, R 2), 1 = lockstep, 2 = eager lockstep (all 16 calls per word,
-// the msc-eager32 ablation; k unused for 1 and 2).
+// Kernel modes: 0 = queue compaction (k = 10 * cap + R, cap in 3..6 unrolled calls, R in
+// {2, 4} x 256 words per workgroup; default cap 4, R 2), 1 = lockstep, 2 = eager lockstep (all
+// 16 calls per word, the msc-eager32 ablation; k unused for 1 and 2).
 static int g_msc_mode = 0;
 static int g_msc_k = 42;
 
@@ -135,24 +142,41 @@ static int msc_mode_ok(int mode, int k)
 {
     if (mode < 0) return ISING_OK;  // process default
     if (mode > 2)
-        return ising_fail(ISING_EINVAL, "msc: mode must be 0 (compaction), 1 (lockstep) or 2 (eager)");
+        return ising_fail(ISING_EINVAL, "msc: mode must be 0 (queue compaction), 1 (lockstep) or 2 (eager)");
     if (mode == 0) {
         const int cap = k / 10, rr = k % 10;
-        if (cap < 1 || cap > 15 || (rr != 1 && rr != 2 && rr != 4))
-            return ising_fail(ISING_EINVAL,
-                              "msc: compaction k = 10 * cap + R with cap in 1..15, R in {1,2,4}");
+        if (cap < 3 || cap > 6 || (rr != 2 && rr != 4))
+            return ising_fail(ISING_EINVAL, "msc: queue compaction k = 10 * cap + R with cap in 3..6, R in {2,4}");
     }
     return ISING_OK;
 }
 
-template <int R>
-static void launch_compact(dim3 grid, hipStream_t st, const MscArgs& a, u64 codes, u32 cap,
-                           unsigned long long* stats)
+template <int R, int CAP, bool ROWW>
+static void launch_queue3(u32 n_tiles, hipStream_t st, const MscArgs& a, u64 codes,
+                          unsigned long long* stats)
 {
     if (stats)
-        hipLaunchKernelGGL((msc_half_compact<R, true>), grid, dim3(256), 0, st, a, codes, cap, stats);
+        hipLaunchKernelGGL((msc_half_queue<R, CAP, ROWW, true>), dim3(n_tiles), dim3(256), 0, st, a, codes, stats);
     else
-        hipLaunchKernelGGL((msc_half_compact<R, false>), grid, dim3(256), 0, st, a, codes, cap, stats);
+        hipLaunchKernelGGL((msc_half_queue<R, CAP, ROWW, false>), dim3(n_tiles), dim3(256), 0, st, a, codes, stats);
+}
+
+template <int R, int CAP>
+static void launch_queue2(u32 n_tiles, hipStream_t st, const MscArgs& a, u64 codes,
+                          unsigned long long* stats)
+{
+    if (a.nw % 64u == 0u) launch_queue3<R, CAP, true>(n_tiles, st, a, codes, stats);
+    else launch_queue3<R, CAP, false>(n_tiles, st, a, codes, stats);
+}
+
+template <int R>
+static void launch_queue(u32 n_tiles, hipStream_t st, const MscArgs& a, u64 codes, u32 cap,
+                         unsigned long long* stats)
+{
+    if (cap == 3) launch_queue2<R, 3>(n_tiles, st, a, codes, stats);
+    else if (cap == 4) launch_queue2<R, 4>(n_tiles, st, a, codes, stats);
+    else if (cap == 5) launch_queue2<R, 5>(n_tiles, st, a, codes, stats);
+    else launch_queue2<R, 6>(n_tiles, st, a, codes, stats);
 }
 
 static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32 t, u32 color,
@@ -181,10 +205,9 @@ static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32
     const u64 codes = msc_level_codes(thr4, thr8);
     if (mode == 0) {
         const u32 cap = (u32)(k / 10), rr = (u32)(k % 10);
-        const dim3 grid((a.n_words + 256u * rr - 1u) / (256u * rr));
-        if (rr == 1) launch_compact<1>(grid, st, a, codes, cap, stats);
-        else if (rr == 2) launch_compact<2>(grid, st, a, codes, cap, stats);
-        else launch_compact<4>(grid, st, a, codes, cap, stats);
+        const u32 n_tiles = (a.n_words + 256u * rr - 1u) / (256u * rr);
+        if (rr == 2) launch_queue<2>(n_tiles, st, a, codes, cap, stats);
+        else launch_queue<4>(n_tiles, st, a, codes, cap, stats);
     } else {
         const dim3 grid((a.n_words + 255u) / 256u);
         if (mode == 1) {

@@ -63,8 +63,11 @@ def test_argument_validation_needs_no_gpu():
     assert b"null" in lib.ising_last_error()
     rc = lib.ising_msc_config(7, 8)
     assert rc == -1 and b"mode" in lib.ising_last_error()
-    rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 0, 43, None, None)
-    assert rc == -1 and b"cap" in lib.ising_last_error()
+    for k in (43, 72, 22, 41, 151):
+        rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 0, k, None, None)
+        assert rc == -1 and b"cap" in lib.ising_last_error()
+    rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 3, 42, None, None)
+    assert rc == -1 and b"mode" in lib.ising_last_error()
     rc = lib.ising_msc_half_ex(1, 1, 15, 1, 0, 0, 0, 0, 0, 0, 0, 42, None, None)
     assert rc == -1 and b"geometry" in lib.ising_last_error()
     rc = lib.ising_split_pack(1, 1, 1, 4, 100, None)

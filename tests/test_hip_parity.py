@@ -37,8 +37,12 @@ def test_int8_bitwise_identical(hip, shape, algorithm):
     assert e == pytest.approx(O.observables_exact(ref.get_lattice())[1] / lat.size)
 
 
-@pytest.mark.parametrize("mode,k", [(0, 42), (0, 11), (0, 31), (0, 24), (0, 151), (1, 0), (2, 0)])
-@pytest.mark.parametrize("shape", [(128, 128), (64, 384), (16, 1024), (130, 256)])
+MSC_MODES = [(0, 42), (0, 32), (0, 52), (0, 62), (0, 44), (0, 34), (1, 0), (2, 0)]
+
+
+@pytest.mark.parametrize("mode,k", MSC_MODES)
+@pytest.mark.parametrize("shape", [(128, 128), (64, 384), (16, 1024), (130, 256),
+                                   (16, 8192), (8, 16384), (6, 24576)])
 def test_msc_bitwise_identical(hip, shape, mode, k):
     lat = O.random_lattice(shape, 9)
     ref = O.MscOracle(shape, 0.44, seed=11, init=lat)
