@@ -171,45 +171,56 @@ __device__ __forceinline__ PhiloxOut msc_philox(const MscArgs& a, const MscCount
 }
 
 // ------------------------------------------------------------------ lazy comparison ----
-constexpr u32 B3_A_ANDN_B = b3imm([](u32 x, u32 y, u32) { return x & ~y; });
-constexpr u32 B3_A_XOR_B_ANDN_C = b3imm([](u32 x, u32 y, u32 z) { return x ^ (y & ~z); });
-constexpr u32 B3_A_B_ANDN_C = b3imm([](u32 x, u32 y, u32 z) { return x & y & ~z; });
-constexpr u32 B3_A_XNOR_BC = b3imm([](u32 x, u32 y, u32 z) { return x & ~(y ^ z); });
-constexpr u32 B3_A_ANDN_B_ANDN_C = b3imm([](u32 x, u32 y, u32 z) { return x & ~y & ~z; });
-constexpr u32 B3_A_AND_B_XOR_C = b3imm([](u32 x, u32 y, u32 z) { return x & (y ^ z); });
-static_assert(B3_A_ANDN_B == 0x30u && B3_A_XOR_B_ANDN_C == 0xB4u && B3_A_B_ANDN_C == 0x40u &&
-                  B3_A_XNOR_BC == 0x90u && B3_A_ANDN_B_ANDN_C == 0x10u && B3_A_AND_B_XOR_C == 0x60u,
-              "bitop3 truth tables");
-
-// One level on a 32-bit half for threshold bits (B4 of thr4, B8 of thr8): a1 lanes compare
-// their uniform's bit with B4, a0 lanes with B8; a lane whose bit differs is decided (u bit 0
-// under a threshold bit 1 accepts, the other way rejects).
+// Both levels of one Philox call in closed form. The threshold bit a lane compares against at
+// one level is 0, 1, a1 (thr4 bit 1, thr8 bit 0) or ~a1 (the reverse): kinds 0-3. With x, y
+// the uniform bits of the two levels: m = und & (x == t_A) stays undecided after level A,
+// und' = m & (y == t_B); the lanes accepted are those decided at a level whose threshold bit
+// is 1 (their uniform bit is then 0). When both levels have t = 1 every decided lane is
+// accepted: pw ^= und ^ und'. One to five v_bitop3 ops per 32-bit half, against two to six
+// for two one-level updates.
 template <u32 B4, u32 B8>
-__device__ __forceinline__ void msc_half_level(u32 lv, u32 a1, u32& pw, u32& und)
+constexpr u32 msc_tkind()
 {
-    if constexpr (B4 && B8) {  // every undecided lane sees a 1 bit
-        pw = bop3(pw, und, lv, B3_A_XOR_B_ANDN_C);
-        und &= lv;
-    } else if constexpr (B4) {  // a1 lanes see 1, a0 lanes 0
-        pw ^= bop3(und, a1, lv, B3_A_B_ANDN_C);
-        und = bop3(und, lv, a1, B3_A_XNOR_BC);
-    } else if constexpr (B8) {  // a0 lanes (und & ~a1) see 1, a1 lanes 0
-        pw ^= bop3(und, a1, lv, B3_A_ANDN_B_ANDN_C);
-        und = bop3(und, lv, a1, B3_A_AND_B_XOR_C);
-    } else {  // every undecided lane sees a 0 bit: a 1 in u rejects
-        und = bop3(und, lv, 0u, B3_A_ANDN_B);
-    }
+    return B4 ? (B8 ? 1u : 2u) : (B8 ? 3u : 0u);
 }
 
-// Both levels of one Philox call. CODE: bit 0 / 1 = thr4 / thr8 bit of level 2 lp, bits 2 / 3
-// the same for level 2 lp + 1.
+template <u32 K>
+__device__ __forceinline__ u32 msc_teq(u32 x, u32 a)  // lanes whose bit equals the threshold bit
+{
+    if constexpr (K == 0) return ~x;
+    else if constexpr (K == 1) return x;
+    else if constexpr (K == 2) return ~(x ^ a);
+    else return x ^ a;
+}
+
+template <u32 K>
+__device__ __forceinline__ u32 msc_tacc(u32 u, u32 x, u32 a)  // lanes of u accepted here
+{
+    if constexpr (K == 0) return 0u;
+    else if constexpr (K == 1) return u & ~x;
+    else if constexpr (K == 2) return u & a & ~x;
+    else return u & ~a & ~x;
+}
+
+template <u32 KA, u32 KB>
+__device__ __forceinline__ void msc_two_levels(u32 x, u32 y, u32 a1, u32& pw, u32& und)
+{
+    const u32 m = und & msc_teq<KA>(x, a1);
+    const u32 u2 = m & msc_teq<KB>(y, a1);
+    if constexpr (KA == 1 && KB == 1) pw ^= und ^ u2;
+    else pw ^= msc_tacc<KA>(und, x, a1) ^ msc_tacc<KB>(m, y, a1);
+    und = u2;
+}
+
+// CODE: bit 0 / 1 = thr4 / thr8 bit of level 2 lp, bits 2 / 3 the same for level 2 lp + 1.
+// Level 2 lp uses r.x (low half) and r.y (high half), level 2 lp + 1 uses r.z and r.w.
 template <u32 CODE>
 __device__ __forceinline__ void msc_pair(const PhiloxOut& r, MscState& s)
 {
-    msc_half_level<CODE & 1u, (CODE >> 1) & 1u>(r.x, s.a1l, s.pwl, s.ul);
-    msc_half_level<CODE & 1u, (CODE >> 1) & 1u>(r.y, s.a1h, s.pwh, s.uh);
-    msc_half_level<(CODE >> 2) & 1u, (CODE >> 3) & 1u>(r.z, s.a1l, s.pwl, s.ul);
-    msc_half_level<(CODE >> 2) & 1u, (CODE >> 3) & 1u>(r.w, s.a1h, s.pwh, s.uh);
+    constexpr u32 KA = msc_tkind<CODE & 1u, (CODE >> 1) & 1u>();
+    constexpr u32 KB = msc_tkind<(CODE >> 2) & 1u, (CODE >> 3) & 1u>();
+    msc_two_levels<KA, KB>(r.x, r.z, s.a1l, s.pwl, s.ul);
+    msc_two_levels<KA, KB>(r.y, r.w, s.a1h, s.pwh, s.uh);
 }
 
 __device__ __forceinline__ void msc_pair_switch(u32 code, const PhiloxOut& r, MscState& s)

@@ -59,14 +59,25 @@ __global__ __launch_bounds__(256) void msc_half_lockstep(MscArgs a, u64 codes,
 // Queue compaction (the default): one workgroup per W = 256 * R words, CAP unrolled lockstep
 // calls, then rounds of one call per packed survivor (msc_queue_tile); streaming stores.
 // R = 2 uses 14 KB of LDS and 36 VGPRs: 8 workgroups (32 waves) per CU.
+static_assert(sizeof(MscQueueLds<1>) <= sizeof(MscQueueLds<2>), "graded tiles share the LDS");
+
+// Graded tail: workgroups [0, n_big) take 256 R-word tiles (XCD band mapped); the ones after,
+// dispatched last, take 256-word tiles (R = 2 only), so the final generation of workgroups
+// finishes sooner and the launch's tail is shorter.
 template <int R, int CAP, bool ROWW, bool STATS>
-__global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u64 codes,
+__global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u64 codes, u32 n_big,
                                                      unsigned long long* stats)
 {
     __shared__ MscQueueLds<R> lds;
-    const u32 base = xcd_band_block(blockIdx.x, gridDim.x) * (256u * R);
     u32 calls = 0, slots = 0, words = 0;
-    msc_queue_tile<R, CAP, ROWW, STATS, false, true>(a, codes, base, lds, calls, slots, words);
+    if (R == 1 || blockIdx.x < n_big) {
+        const u32 base = xcd_band_block(blockIdx.x, n_big) * (256u * R);
+        msc_queue_tile<R, CAP, ROWW, STATS, false, true>(a, codes, base, lds, calls, slots, words);
+    } else if constexpr (R == 2) {
+        const u32 base = n_big * 512u + (blockIdx.x - n_big) * 256u;
+        msc_queue_tile<1, CAP, ROWW, STATS, false, true>(
+            a, codes, base, *reinterpret_cast<MscQueueLds<1>*>(&lds), calls, slots, words);
+    }
     if (STATS) msc_stats(stats, calls, slots, words);
 }
 
@@ -155,10 +166,19 @@ template <int R, int CAP, bool ROWW>
 static void launch_queue3(u32 n_tiles, hipStream_t st, const MscArgs& a, u64 codes,
                           unsigned long long* stats)
 {
+    // R = 2: the last min(1024, a quarter of the words / 256) tiles are 256-word tiles
+    u32 n_big = n_tiles, grid = n_tiles;
+    if (R == 2) {
+        u32 small = a.n_words / 1024u;  // 256-word tiles in a quarter of the words
+        if (small > 1024u) small = 1024u;
+        small &= ~1u;
+        n_big = (a.n_words - small * 256u) / 512u;
+        grid = n_big + (a.n_words - n_big * 512u + 255u) / 256u;
+    }
     if (stats)
-        hipLaunchKernelGGL((msc_half_queue<R, CAP, ROWW, true>), dim3(n_tiles), dim3(256), 0, st, a, codes, stats);
+        hipLaunchKernelGGL((msc_half_queue<R, CAP, ROWW, true>), dim3(grid), dim3(256), 0, st, a, codes, n_big, stats);
     else
-        hipLaunchKernelGGL((msc_half_queue<R, CAP, ROWW, false>), dim3(n_tiles), dim3(256), 0, st, a, codes, stats);
+        hipLaunchKernelGGL((msc_half_queue<R, CAP, ROWW, false>), dim3(grid), dim3(256), 0, st, a, codes, n_big, stats);
 }
 
 template <int R, int CAP>
