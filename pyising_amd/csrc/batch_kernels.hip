@@ -114,29 +114,37 @@ __device__ __forceinline__ u32 batch_msc_tile(const MscArgs& a, u64 codes, u32 b
     return msc_queue_tile<2, 4, false, false, true, true>(a, codes, base, lds, calls, slots, words);
 }
 
+// One int8 tile; thr_lds is this workgroup's threshold table for the lattice (filled by
+// batch_int8 below, which all threads call).
 template <int NDIM, int HB, int V>
-__device__ __forceinline__ u32 batch_int8_tile(const BatchDesc& d, u32 local, u32 t, u32 color)
+__device__ __forceinline__ u32 batch_int8_tile(const BatchDesc& d, const u32* thr_lds, u32 local,
+                                               u32 t, u32 color)
 {
     const u32 ci = local * 256u + threadIdx.x;
     if (ci >= d.g.n_chunks) return 0u;
     int8_t* own = (int8_t*)(color ? d.c1 : d.c0);
     const int8_t* other = (const int8_t*)(color ? d.c0 : d.c1);
-    const u32 minus = int8_update_chunk<NDIM, HB, V, true>(own, other, d.g, d.tb, t, color, d.k0,
-                                                            d.k1, ci);
+    const u32 minus = int8_update_chunk<NDIM, HB, V, true>(own, other, d.g, thr_lds, t, color,
+                                                            d.k0, d.k1, ci);
     return (u32)V - minus;
 }
 
 template <int NDIM, int HB>
-__device__ __forceinline__ u32 batch_int8_v(const BatchDesc& d, u32 local, u32 t, u32 color)
+__device__ __forceinline__ u32 batch_int8_v(const BatchDesc& d, u32* thr_lds, u32 local, u32 t,
+                                            u32 color)
 {
-    if (d.vec == 16) return batch_int8_tile<NDIM, HB, 16>(d, local, t, color);
-    if (d.vec == 8) return batch_int8_tile<NDIM, HB, 8>(d, local, t, color);
-    return batch_int8_tile<NDIM, HB, 4>(d, local, t, color);
+    __syncthreads();  // the previous tile's readers of thr_lds are done
+    int8_fill_table<NDIM, HB>(thr_lds, d.tb);
+    __syncthreads();
+    if (d.vec == 16) return batch_int8_tile<NDIM, HB, 16>(d, thr_lds, local, t, color);
+    if (d.vec == 8) return batch_int8_tile<NDIM, HB, 8>(d, thr_lds, local, t, color);
+    return batch_int8_tile<NDIM, HB, 4>(d, thr_lds, local, t, color);
 }
 
 __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
 {
     __shared__ MscQueueLds<2> lds;
+    __shared__ u32 thr_lds[8];
     cg::grid_group grid = cg::this_grid();
     const u32 done0 = (u32)*ka.sweeps_done;
     const u32 gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,10 +160,10 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
                 const u32 t = d.t0 + sw;
                 u32 plus = 0;
                 if (d.kind == BATCH_INT8) {
-                    if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1>(d, local, t, color)
-                                                 : batch_int8_v<2, 0>(d, local, t, color);
-                    else plus = d.hb ? batch_int8_v<3, 1>(d, local, t, color)
-                                     : batch_int8_v<3, 0>(d, local, t, color);
+                    if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1>(d, thr_lds, local, t, color)
+                                                 : batch_int8_v<2, 0>(d, thr_lds, local, t, color);
+                    else plus = d.hb ? batch_int8_v<3, 1>(d, thr_lds, local, t, color)
+                                     : batch_int8_v<3, 0>(d, thr_lds, local, t, color);
                 } else {
                     MscArgs a;
                     a.own = (u64*)(color ? d.c1 : d.c0);
@@ -201,6 +209,7 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
 __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
 {
     __shared__ MscQueueLds<2> lds;
+    __shared__ u32 thr_lds[8];
     const u32 tile = xcd_band_block(blockIdx.x, gridDim.x);
     const u32 sw = (u32)*ka.sweeps_done;
     const u32 par = sw & 1u;
@@ -210,10 +219,10 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
     const u32 t = d.t0 + sw;
     u32 plus = 0;
     if (d.kind == BATCH_INT8) {
-        if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1>(d, local, t, color)
-                                     : batch_int8_v<2, 0>(d, local, t, color);
-        else plus = d.hb ? batch_int8_v<3, 1>(d, local, t, color)
-                         : batch_int8_v<3, 0>(d, local, t, color);
+        if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1>(d, thr_lds, local, t, color)
+                                     : batch_int8_v<2, 0>(d, thr_lds, local, t, color);
+        else plus = d.hb ? batch_int8_v<3, 1>(d, thr_lds, local, t, color)
+                         : batch_int8_v<3, 0>(d, thr_lds, local, t, color);
     } else {
         MscArgs a;
         a.own = (u64*)(color ? d.c1 : d.c0);

@@ -67,37 +67,42 @@ __device__ __forceinline__ u32 byte_sum(u32 v)  // sum of the 4 bytes (each <= 6
     return (v * LOW_BITS) >> 24;
 }
 
-template <int NDIM>
-__device__ __forceinline__ u32 metro_thr(u32 a, const Int8Table& tb)
+// Per-workgroup threshold table in LDS, indexed by the count byte 4k (byte offset = 4k):
+// Metropolis, k = anti-aligned neighbours: table[2d - k] for k < d (k >= d always flips);
+// heat bath, k = -1 neighbours: table[2d - k]. Threads 0..7 write it; callers put a barrier
+// between this and the first int8_update_chunk.
+template <int NDIM, int HB>
+__device__ __forceinline__ void int8_fill_table(u32* thr_lds, const Int8Table& tb)
 {
-    // only a < d can be accepted by the comparison; a >= d always flips
-    if constexpr (NDIM == 2) {
-        return a == 0 ? tb.v[4] : tb.v[3];
-    } else {
-        return a == 0 ? tb.v[6] : (a == 1 ? tb.v[5] : tb.v[4]);
+    constexpr u32 Z = 2 * NDIM;
+    const u32 k = threadIdx.x;
+    if (k < 8u) {
+        u32 v = 0;
+#pragma unroll
+        for (u32 i = 0; i <= Z; ++i)
+            if (Z - i == k && (HB || k < (u32)NDIM)) v = tb.v[i];
+        thr_lds[k] = v;
     }
 }
 
-template <int NDIM>
-__device__ __forceinline__ u32 heat_thr(u32 m, const Int8Table& tb)
-{
-    constexpr int Z = 2 * NDIM;
-    u32 thr = tb.v[Z];
-#pragma unroll
-    for (int k = 1; k <= Z; ++k) thr = (m == (u32)k) ? tb.v[Z - k] : thr;
-    return thr;
-}
-
-
 // Update the V consecutive colour sites of chunk ci (one row segment) of colour `color`.
+// Per 4-site word: one Philox call (scalar-uniform rounds: counter words 1-3 are sweep, colour
+// and 0); neighbour counts as bytes 4k from (s ^ n) & 0x04040404 (Metropolis: anti-aligned;
+// spins are 0x01 / 0xFF, so bit 2 differs exactly for opposite spins) or n & 0x04040404 (heat
+// bath: -1 neighbours); each byte addresses its site's threshold in the LDS table; the four
+// borrows of u - threshold accumulate into a 4-bit vector, spread to bytes by one multiply
+// ((b * 0x204081) & 0x01010101 moves bit b to byte b). Metropolis adds the forced flips
+// (count >= d) by a bytewise compare and flips with s ^ 0xFE; heat bath writes 0x01 or 0xFF.
 // Returns the number of -1 spins written (for magnetisation sums when COUNT).
 template <int NDIM, int HB, int V, bool COUNT>
 __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
                                                  const int8_t* __restrict__ other, const Int8Geom& g,
-                                                 const Int8Table& tb, u32 t, u32 color, u32 k0,
+                                                 const u32* thr_lds, u32 t, u32 color, u32 k0,
                                                  u32 k1, u32 ci)
 {
     constexpr int NWd = V / 4;
+    constexpr u32 K4 = 0x04040404u;
+    constexpr u32 FORCE = (128u - 4u * NDIM) * LOW_BITS;  // byte >= 4d sets its bit 7
     const u32 row = fast_div(ci, g.div_chunks);
     const u32 j0 = (ci - row * g.chunks_row) * V;
     const u32 z = fast_div(row, g.div_ly);
@@ -127,50 +132,46 @@ __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
 #pragma unroll
         for (int i = 0; i < NWd; ++i) {
             const u32 nxt = (i + 1 < NWd) ? C.w[i + 1 < NWd ? i + 1 : 0] : e;
-            H.w[i] = (C.w[i] >> 8) | (nxt << 24);
+            H.w[i] = __builtin_amdgcn_alignbit(nxt, C.w[i], 8u);
         }
     } else {
 #pragma unroll
         for (int i = 0; i < NWd; ++i) {
-            const u32 prv = (i > 0) ? (C.w[i > 0 ? i - 1 : 0] >> 24) : e;
-            H.w[i] = (C.w[i] << 8) | prv;
+            const u32 prv = (i > 0) ? C.w[i > 0 ? i - 1 : 0] : (e << 24);
+            H.w[i] = __builtin_amdgcn_alignbit(C.w[i], prv, 24u);
         }
     }
 
+    const uint8_t* thr_bytes = reinterpret_cast<const uint8_t*>(thr_lds);
     Vec<V> O;
     u32 minus = 0;
 #pragma unroll
     for (int i = 0; i < NWd; ++i) {
         const u32 q = (u32)((n0 >> 2) + (u64)i);  // quad index, truncated to 32 bits
-        const PhiloxOut r = philox4x32_10(q, t, color, 0u, k0, k1);
+        const u64 qm = (u64)q * PHILOX_M0;
+        const PhiloxOut r = philox_lane((u32)(qm >> 32), (u32)qm, t, color, 0u, k0, k1);
         const u32 uu[4] = {r.x, r.y, r.z, r.w};
         u32 cnt;
         if constexpr (HB) {
-            cnt = ((C.w[i] >> 1) & LOW_BITS) + ((H.w[i] >> 1) & LOW_BITS) +
-                  ((U.w[i] >> 1) & LOW_BITS) + ((D.w[i] >> 1) & LOW_BITS);
-            if constexpr (NDIM == 3)
-                cnt += ((ZU.w[i] >> 1) & LOW_BITS) + ((ZD.w[i] >> 1) & LOW_BITS);
+            cnt = (C.w[i] & K4) + (H.w[i] & K4) + (U.w[i] & K4) + (D.w[i] & K4);
+            if constexpr (NDIM == 3) cnt += (ZU.w[i] & K4) + (ZD.w[i] & K4);
         } else {
             const u32 s = S.w[i];
-            cnt = (((s ^ C.w[i]) >> 1) & LOW_BITS) + (((s ^ H.w[i]) >> 1) & LOW_BITS) +
-                  (((s ^ U.w[i]) >> 1) & LOW_BITS) + (((s ^ D.w[i]) >> 1) & LOW_BITS);
-            if constexpr (NDIM == 3)
-                cnt += (((s ^ ZU.w[i]) >> 1) & LOW_BITS) + (((s ^ ZD.w[i]) >> 1) & LOW_BITS);
+            cnt = ((s ^ C.w[i]) & K4) + ((s ^ H.w[i]) & K4) + ((s ^ U.w[i]) & K4) +
+                  ((s ^ D.w[i]) & K4);
+            if constexpr (NDIM == 3) cnt += ((s ^ ZU.w[i]) & K4) + ((s ^ ZD.w[i]) & K4);
         }
-        u32 out = 0;
+        u32 bits = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const u32 c = (cnt >> (8 * b)) & 0xFFu;
-            if constexpr (HB) {
-                const bool up = uu[b] < heat_thr<NDIM>(c, tb);
-                out |= (up ? 0x01u : 0xFFu) << (8 * b);
-            } else {
-                const bool flip = (c >= (u32)NDIM) || (uu[b] < metro_thr<NDIM>(c, tb));
-                out |= (flip ? 0xFEu : 0x00u) << (8 * b);
-            }
+        for (int b = 3; b >= 0; --b) {
+            const u32 thr = *reinterpret_cast<const u32*>(thr_bytes + ((cnt >> (8 * b)) & 0xFFu));
+            bits = bits + bits + (uu[b] < thr ? 1u : 0u);
         }
-        O.w[i] = HB ? out : (S.w[i] ^ out);
-        if constexpr (COUNT) minus += byte_sum((O.w[i] >> 1) & LOW_BITS);
+        u32 f01 = (bits * 0x204081u) & LOW_BITS;
+        if constexpr (!HB) f01 |= ((cnt + FORCE) >> 7) & LOW_BITS;
+        const u32 m = (f01 << 8) - (f01 << 1);  // 0xFE in every flagged byte
+        O.w[i] = HB ? ~m : (S.w[i] ^ m);
+        if constexpr (COUNT) minus += __popc(O.w[i] & 0x80808080u);
     }
     store_vec<V>(own + n0, O);
     return minus;

@@ -21,8 +21,8 @@
 // Decision rules (src/pyising/backends/numpy_backend.py:35-44, src/pyising/lattice.py:128-142):
 //   Metropolis, a = anti-aligned neighbours: flip iff a >= d or u < table[2d - a]
 //   heat bath, m = -1 neighbours:            new spin +1 iff u < table[2d - m]
-// The table is passed by value (scalar registers); per-site selection is a short v_cndmask
-// chain, never an indexed memory load.
+// The table is passed by value and copied into an 8-entry LDS table per workgroup, indexed
+// directly by the SWAR count byte (see int8_update_chunk).
 #include "int8_core.hpp"
 
 // One thread: sites j0 .. j0 + V - 1 of one row.
@@ -32,10 +32,13 @@ __global__ __launch_bounds__(256) void int8_half_kernel(int8_t* __restrict__ own
                                                         Int8Geom g, Int8Table tb, u32 t, u32 color,
                                                         u32 k0, u32 k1)
 {
+    __shared__ u32 thr_lds[8];
+    int8_fill_table<NDIM, HB>(thr_lds, tb);
+    __syncthreads();
     const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
     const u32 ci = blk * blockDim.x + threadIdx.x;
     if (ci >= g.n_chunks) return;
-    int8_update_chunk<NDIM, HB, V, false>(own, other, g, tb, t, color, k0, k1, ci);
+    int8_update_chunk<NDIM, HB, V, false>(own, other, g, thr_lds, t, color, k0, k1, ci);
 }
 
 
