@@ -6,8 +6,8 @@ One step = one full checkerboard sweep (two half-sweep launches) of the 16384^2 
 (beta = 1/T_c exact, seed 1, hot start = numpy default_rng(1) draws), resident in HBM. Prints
 one JSON line: flips/ns (sweeps x spins / s / 1e9), the roofline of the dominant kernel (the
 msc half sweep) under the 0.375 B/update traffic model against the 8 TB/s HBM peak, the
-measured copy bandwidth and Philox ceiling, the batched-scan comparison when available, and
-the oracle timed on one host core as the CPU baseline.
+measured copy bandwidth and Philox ceiling, the batched-scan comparison, the serving loop on the
+Poisson request stream, and the oracle timed on one host core as the CPU baseline.
 """
 
 from __future__ import annotations
@@ -107,6 +107,26 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
             "binder_L4096_Tmid": round(mom[-9].binder, 4)}
 
 
+def serving(n_requests: int = 300, rate: float = 200.0, seed: int = 0) -> dict:
+    """The serving loop on the config-5 request stream: Poisson arrivals at ``rate`` per second
+    (2D L = 32..4096 int8/msc with p = 0.7, else 3D L = 16..256 int8 Metropolis/heat bath;
+    T in [0.8, 1.2] T_c; sweeps log-uniform in [100, 10^4]; hot starts), served in real time
+    with continuous batching. Latency = finish - arrival; aggregate flips/ns over the wall time
+    of the whole stream (requests arrive over ~1.5 s)."""
+    from pyising_amd.serve import Server, poisson_requests
+
+    reqs = poisson_requests(n_requests, rate=rate, seed=seed)
+    work = sum(r.sweeps * r.n_spins for r in reqs)
+    srv = Server(budget_bytes=8 << 30, max_slots=256, step_sweeps=25, max_spins=1 << 24)
+    rep = srv.run(reqs, realtime=True, timeout=600)
+    srv.close()
+    out = rep.summary()
+    out.update({"rate_per_s": rate, "arrival_span_s": round(reqs[-1].arrival, 3),
+                "offered_flips_per_ns": round(work / reqs[-1].arrival / 1e9, 1),
+                "stream": f"poisson_requests(n={n_requests}, rate={rate}, seed={seed})"})
+    return out
+
+
 def traffic_from_profile() -> int | None:
     """HBM bytes per msc half-sweep launch from the committed rocprofv3 PMC summary."""
     try:
@@ -124,6 +144,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ceilings", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path "
@@ -210,6 +231,8 @@ def main() -> int:
         }
     if not args.no_batch:
         result["batched_scan"] = batch_vs_plain()
+    if not args.no_serve:
+        result["serving"] = serving()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)

@@ -86,6 +86,12 @@ int ising_split_pack(const void* d_full, uint64_t* d_w0, uint64_t* d_w1, long lo
 /* Inverse: unpack_bits + from_compressed, src/pyising/lattice.py:63-72,85-90. */
 int ising_unpack_merge(const uint64_t* d_w0, const uint64_t* d_w1, void* d_full, long long rows,
                        int lx, void* stream);
+/* Hot start on the device: n bytes (n % 8 == 0, 8-byte aligned) equal to numpy
+ * default_rng(seed).integers(0, 2, n, int8), given the seeded PCG64 (state, inc) that numpy
+ * reports for default_rng(seed) (split into 64-bit halves). Replaces the host draws of
+ * random_lattice, src/pyising/lattice.py:93-96; the split kernels then map 0/1 to -1/+1. */
+int ising_hot_start(void* d_out, long long n, uint64_t state_hi, uint64_t state_lo,
+                    uint64_t inc_hi, uint64_t inc_lo, void* stream);
 /* Fill n uint64 words with `value` (cold starts, counters): _fill_cold of
  * src/pyising/backends/base.py:119-120, cupy_backend.py:154-156. */
 int ising_fill_u64(uint64_t* d_dst, long long n, uint64_t value, void* stream);

@@ -47,6 +47,7 @@ SIGNATURES: dict[str, list] = {
     "ising_philox_rate": [_P, _I, _I, _U32, _U32, _P],
     "ising_copy": [_P, _P, _SZ, _I, _P],
     "ising_fill_u64": [_P, _LL, _U64, _P],
+    "ising_hot_start": [_P, _LL, _U64, _U64, _U64, _U64, _P],
     "ising_split_int8": [_P, _P, _P, _LL, _I, _I, _P],
     "ising_merge_int8": [_P, _P, _P, _LL, _I, _I, _P],
     "ising_split_pack": [_P, _P, _P, _LL, _I, _P],

@@ -5,9 +5,9 @@ Drop-in for the reference's GPU variants (src/pyising/backends/cupy_backend.py:4
 ``HipMultiSpin`` ~ ``CupyMultiSpin`` (64 spins per uint64, 2D Metropolis, lazy comparison),
 ``HipMultiSpinEager`` ~ ``CupyMultiSpinEager`` (all 32 bits always drawn).
 
-Differences by design: the hot start reproduces numpy ``default_rng(seed)`` draws exactly
-(the reference's GPU backends use other generators there), full-lattice <-> colour-array
-conversions run on the GPU, ``sweep(n)`` issues all 2n launches from C, and observables are
+Differences by design: the hot start reproduces numpy ``default_rng(seed)`` draws exactly,
+generated on the device from numpy's seeded PCG64 state (the reference's GPU backends use other
+generators there), full-lattice <-> colour-array conversions run on the GPU, ``sweep(n)`` issues all 2n launches from C, and observables are
 exact integer reductions on the device.
 """
 
@@ -16,7 +16,7 @@ from __future__ import annotations
 import numpy as np
 
 from .. import _lib
-from ..lattice import hot_start_bits, split_seed
+from ..lattice import pcg64_state, split_seed
 from .base import IsingSimulation
 
 _ONES_INT8 = 0x0101010101010101
@@ -69,6 +69,18 @@ class _HipBase(IsingSimulation):
         buf.upload(full, self.stream_ptr)
         return buf, tmp
 
+    def _hot_start_full(self):
+        """The hot start's {0, 1} draws (numpy ``default_rng(seed).integers(0, 2, shape, int8)``,
+        src/pyising/lattice.py:93-96) generated on the device into a full-lattice buffer:
+        the host only seeds numpy's PCG64; returns (buffer, temporary?)."""
+        n = self.n_spins
+        buf, tmp = self._full_buffer(n)
+        st, inc = pcg64_state(self.seed)
+        m64 = (1 << 64) - 1
+        _lib.check(self._L.ising_hot_start(buf.ptr, n, st >> 64, st & m64, inc >> 64, inc & m64,
+                                           self.stream_ptr), "hot_start")
+        return buf, tmp
+
     def _hip_setup(self) -> None:
         self._L = _lib.lib()
         self.k0, self.k1 = split_seed(self.seed)
@@ -108,8 +120,7 @@ class HipInt8(_HipBase):
     def _upload_table(self) -> None:
         self._table_arr = np.ascontiguousarray(self.table, dtype=np.uint32)
 
-    def _split_from_host(self, full: np.ndarray) -> None:
-        buf, tmp = self._upload_full(full)
+    def _split(self, buf, tmp: bool) -> None:
         _lib.check(self._L.ising_split_int8(buf.ptr, self.c[0].ptr, self.c[1].ptr, self.rows,
                                             self.ly, self.shape[-1], self.stream_ptr),
                    "split_int8")
@@ -118,10 +129,10 @@ class HipInt8(_HipBase):
             buf.free()
 
     def set_lattice(self, lat: np.ndarray) -> None:
-        self._split_from_host(_full_lattice(lat, self.shape))
+        self._split(*self._upload_full(_full_lattice(lat, self.shape)))
 
     def _fill_hot(self) -> None:
-        self._split_from_host(hot_start_bits(self.shape, self.seed))
+        self._split(*self._hot_start_full())
 
     def _fill_cold(self) -> None:
         n = self.rows * self.lh // 8
@@ -183,8 +194,7 @@ class HipMultiSpin(_HipBase):
         self.w = [self._alloc(8 * self.n_words), self._alloc(8 * self.n_words)]
         self.stats_buf = None
 
-    def _split_from_host(self, full: np.ndarray) -> None:
-        buf, tmp = self._upload_full(full)
+    def _split(self, buf, tmp: bool) -> None:
         _lib.check(self._L.ising_split_pack(buf.ptr, self.w[0].ptr, self.w[1].ptr, self.rows,
                                             self.shape[-1], self.stream_ptr), "split_pack")
         self.synchronize()
@@ -192,10 +202,10 @@ class HipMultiSpin(_HipBase):
             buf.free()
 
     def set_lattice(self, lat: np.ndarray) -> None:
-        self._split_from_host(_full_lattice(lat, self.shape))
+        self._split(*self._upload_full(_full_lattice(lat, self.shape)))
 
     def _fill_hot(self) -> None:
-        self._split_from_host(hot_start_bits(self.shape, self.seed))
+        self._split(*self._hot_start_full())
 
     def _fill_cold(self) -> None:
         for buf in self.w:

@@ -120,6 +120,62 @@ __global__ __launch_bounds__(256) void fill_u64_kernel(u64* __restrict__ p, u64 
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// ----------------------------------------------------------------------------- hot start ----
+// numpy's default_rng(seed).integers(0, 2, n, int8) on the device (the reference's hot start,
+// src/pyising/lattice.py:93-96). The bit generator is PCG64, PCG XSL-RR 128/64 (O'Neill 2014):
+// state <- state * M + inc mod 2^128, output rotr64(hi ^ lo, state >> 122) of the new state.
+// For range 2, numpy's int8 path takes bytes of the 64-bit outputs in little-endian order and
+// keeps byte >> 7, so output byte i is bit 8 (i % 8) + 7 of draw i / 8. The host passes the
+// seeded (state, inc) numpy computes; each thread jumps ahead (LCG advance by squaring, Brown
+// 1994) to its first draw and writes HOT_DRAWS draws as 8 bytes each.
+#define HOT_DRAWS 16u
+
+struct U128 {
+    u64 hi, lo;
+};
+
+__device__ __forceinline__ U128 mul128(U128 a, U128 b)  // mod 2^128
+{
+    return U128{__umul64hi(a.lo, b.lo) + a.lo * b.hi + a.hi * b.lo, a.lo * b.lo};
+}
+
+__device__ __forceinline__ U128 add128(U128 a, U128 b)
+{
+    const u64 lo = a.lo + b.lo;
+    return U128{a.hi + b.hi + (lo < a.lo ? 1ull : 0ull), lo};
+}
+
+__device__ __forceinline__ u64 pcg_output(U128 s)
+{
+    const u64 x = s.hi ^ s.lo;
+    const u32 r = (u32)(s.hi >> 58);
+    return (x >> r) | (x << ((64u - r) & 63u));
+}
+
+__global__ __launch_bounds__(256) void hot_start_kernel(u64* __restrict__ out, u64 n_draws,
+                                                        U128 state, U128 inc)
+{
+    const U128 mult = U128{0x2360ED051FC65DA4ull, 0x4385DF649FCCF645ull};
+    const u64 first = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * HOT_DRAWS;
+    if (first >= n_draws) return;
+    // advance `state` by `first` steps
+    U128 cur_mult = mult, cur_plus = inc, acc_mult = U128{0, 1}, acc_plus = U128{0, 0};
+    for (u64 d = first; d; d >>= 1) {
+        if (d & 1) {
+            acc_mult = mul128(acc_mult, cur_mult);
+            acc_plus = add128(mul128(acc_plus, cur_mult), cur_plus);
+        }
+        cur_plus = mul128(add128(cur_mult, U128{0, 1}), cur_plus);
+        cur_mult = mul128(cur_mult, cur_mult);
+    }
+    U128 s = add128(mul128(acc_mult, state), acc_plus);
+    const u64 last = first + HOT_DRAWS < n_draws ? first + HOT_DRAWS : n_draws;
+    for (u64 k = first; k < last; ++k) {
+        s = add128(mul128(s, mult), inc);
+        out[k] = (pcg_output(s) >> 7) & 0x0101010101010101ull;
+    }
+}
+
 static unsigned grid_for(u64 threads)
 {
     return (unsigned)((threads + 255) / 256);
@@ -176,6 +232,22 @@ int ising_unpack_merge(const uint64_t* w0, const uint64_t* w1, void* full, long 
     hipLaunchKernelGGL(unpack_merge_kernel, dim3(grid_for(nwords * 64)), dim3(256), 0,
                        as_stream(stream), w0, w1, (uint8_t*)full, (u32)rows, (u32)(lx / 128));
     ISING_LAUNCHED("unpack_merge_kernel");
+    return ISING_OK;
+}
+
+// n bytes (n % 8 == 0) of numpy default_rng(seed).integers(0, 2, n, int8), from the seeded
+// PCG64 (state, inc) that numpy reports (PCG64(seed).state). Stream-ordered.
+int ising_hot_start(void* d_out, long long n, uint64_t state_hi, uint64_t state_lo,
+                    uint64_t inc_hi, uint64_t inc_lo, void* stream)
+{
+    if (!d_out || n < 0 || (n & 7)) return ising_fail(ISING_EINVAL, "hot_start: null buffer or n % 8 != 0");
+    if (((uintptr_t)d_out & 7u) != 0) return ising_fail(ISING_EINVAL, "hot_start: buffer not 8-byte aligned");
+    if (n == 0) return ISING_OK;
+    const u64 draws = (u64)n / 8u;
+    const u64 threads = (draws + HOT_DRAWS - 1u) / HOT_DRAWS;
+    hipLaunchKernelGGL(hot_start_kernel, dim3(grid_for(threads)), dim3(256), 0, as_stream(stream),
+                       (u64*)d_out, draws, U128{state_hi, state_lo}, U128{inc_hi, inc_lo});
+    ISING_LAUNCHED("hot_start_kernel");
     return ISING_OK;
 }
 

@@ -69,6 +69,13 @@ def threshold_table(beta: float, ndim: int, algorithm: str) -> np.ndarray:
     return np.array(vals, dtype=np.uint32)
 
 
+def pcg64_state(seed: int) -> tuple[int, int]:
+    """The seeded (state, inc) of numpy's PCG64 behind ``default_rng(seed)`` (numpy's own
+    SeedSequence seeding); the device hot-start kernel continues this stream."""
+    st = np.random.PCG64(int(seed) & 0xFFFFFFFFFFFFFFFF).state["state"]
+    return int(st["state"]), int(st["inc"])
+
+
 def hot_start_bits(shape, seed: int) -> np.ndarray:
     """The hot start's random draws: ``numpy.random.default_rng(seed).integers(0, 2, shape, int8)``.
 

@@ -5,11 +5,12 @@ A request asks for ``sweeps`` sweeps of one lattice (2D or 3D, int8 or msc, a te
 seed, a hot start) and gets back the exact final (M, E). Design:
 
 * Memory. The server makes one device allocation for request state (``budget_bytes``) and one
-  staging buffer for host uploads at start-up, and nothing after that: a host-side first-fit
+  staging buffer (hot starts are generated there on the device) at start-up, and nothing after
+  that: a host-side first-fit
   arena (:class:`DeviceArena`) hands every admitted request one region for its colour arrays and
   reduction scratch, returned when the request finishes. Admission is by that footprint, so a
   request is admitted exactly when its state fits the free memory and a batch slot is free.
-* Streams. Admissions (hot-start upload, split into colour arrays) and result reads run on a
+* Streams. Admissions (device hot start, split into colour arrays) and result reads run on a
   setup stream; batched sweeps run on a work stream. Both are non-blocking streams, and the loop
   calls no hipMalloc/hipFree, so admission work overlaps the sweeps on the device.
 * Continuous batching. Every step sweeps all active lattices ``step_sweeps`` times (fewer if a
@@ -413,7 +414,10 @@ class Server:
             M, E = a.sim.observables_exact()
             lat = a.sim.get_lattice() if keep_lattices else None
             r = a.req
-            rep.results.append(Result(r.rid, M, E, r.n_spins, r.sweeps, r.arrival, a.admitted,
+            # latency counts from the request's arrival; in batch mode every request is
+            # submitted at time 0
+            arrival = r.arrival if realtime else 0.0
+            rep.results.append(Result(r.rid, M, E, r.n_spins, r.sweeps, arrival, a.admitted,
                                       clock(), lat))
             self.ctrl.release(rid)
 

@@ -93,6 +93,17 @@ def test_hot_and_cold_starts(variant):
     assert abs(m) < 0.05 and abs(e) < 0.05
 
 
+@pytest.mark.parametrize("variant,shape", [("int8", (64, 48)), ("int8", (8, 6, 16)),
+                                           ("msc", (64, 256)), ("msc", (130, 384))])
+@pytest.mark.parametrize("seed", [0, 7, 2**63 + 5, 2**64 - 1])
+def test_device_hot_start_equals_numpy_draws(variant, shape, seed):
+    """The device-generated hot start is numpy default_rng(seed)'s draws mapped 0 -> -1."""
+    from pyising_amd import create
+
+    sim = create("hip", variant, shape=shape, beta=0.4, seed=seed, init="hot")
+    np.testing.assert_array_equal(sim.get_lattice(), O.random_lattice(shape, seed))
+
+
 def test_cold_start_3d(hip):
     sim = hip.HipInt8((8, 8, 16), 0.3, init="cold")
     assert sim.observables_exact() == (1024, -3 * 1024)
