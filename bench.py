@@ -67,20 +67,22 @@ def cpu_baseline(seconds: float = 10.0) -> dict:
 def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
     """Binder temperature scan: 96 int8 lattices (L = 64..2048) and 32 msc lattices
     (L = 1024, 4096), 16 temperatures in [2.20, 2.34] each, seeds base + index, hot starts.
-    Aggregate flips/ns of the batched scan (all lattices per launch, hipGraph replays) versus
-    the plain per-lattice loop (``sim.sweep(n)`` lattice after lattice)."""
+    Aggregate flips/ns of the batched scan (all lattices per launch; tiles, launch form and graph
+    length picked by the autotuner, its timing outside the measured region) versus the plain
+    per-lattice loop (``sim.sweep(n)`` lattice after lattice)."""
     from pyising_amd import _lib
     from pyising_amd.batch import BatchScan, temperature_scan_lattices
 
     sims = temperature_scan_lattices()
     n_spins = sum(s.n_spins for s in sims)
-    scan = BatchScan(sims)
-    per_replay = 50
-    graph = scan.capture(per_replay)
+    scan = BatchScan.tuned(sims)  # autotuner: msc tile per L, launch form, graph length
+    per_replay = scan.graph_sweeps
+    graph = per_replay > 0
     e0, e1 = _lib.Event(), _lib.Event()
     if graph:
         scan.replay(1)
     else:
+        per_replay = 50
         scan.sweep(per_replay)
     scan.synchronize()
     reps = max(1, batch_sweeps // per_replay)
@@ -92,6 +94,7 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
     e1.record(scan.stream.ptr)
     batch_rate = reps * per_replay * n_spins / (e0.elapsed_ms(e1) * 1e-3) / 1e9
     mom = scan.moments()
+    tuning = scan.tuning
     scan.close()
     for s in sims:
         s.sweep(1)
@@ -104,6 +107,8 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
     return {"lattices": len(sims), "spins": int(n_spins), "graph": bool(graph),
             "batched_flips_per_ns": round(batch_rate, 1), "plain_flips_per_ns": round(plain_rate, 1),
             "speedup": round(batch_rate / plain_rate, 2),
+            "tuning": {"msc_tiles": {"x".join(map(str, k)): v for k, v in tuning.msc_tiles.items()},
+                       "persistent": tuning.persistent, "graph_sweeps": tuning.graph_sweeps},
             "binder_L4096_Tmid": round(mom[-9].binder, 4)}
 
 

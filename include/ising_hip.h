@@ -154,7 +154,9 @@ int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, in
  * lattice: sum M, sum |M|, sum M^2 (int64) and sum M^4 (128-bit) for Binder cumulants
  * (src/pyising/observables.py:39-42). */
 typedef struct {
-    int kind;            /* 0 = int8 compressed checkerboard, 1 = msc packed */
+    int kind;            /* 0 = int8 compressed checkerboard; msc packed: 1 = the batch's
+                          * default tile (msc_mode), 2 = lockstep 256-word tiles, 3 = queue
+                          * compaction 512-word tiles */
     int ndim;            /* 2 or 3 (msc: 2) */
     int heatbath;        /* int8 only: 0 Metropolis, 1 heat bath */
     int lz, ly, lx;      /* shape; lz = 1 in 2D */

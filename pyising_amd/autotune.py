@@ -1,0 +1,159 @@
+"""Autotuner for batched scans (no reference counterpart).
+
+A batch's speed depends on choices that never change its results: the tile of each msc
+lattice size (queue compaction over 512 words, or lockstep over 256 words; small lattices
+have too few words to fill waves after compaction), the launch form (one launch per half
+sweep, or one persistent cooperative launch), and the number of sweeps per captured graph.
+:func:`tune_batch` times candidates on scratch lattices of the same shapes, temperatures and
+algorithms (hot starts from other seeds, so the callers' lattices are untouched), by
+coordinate descent: the msc tile per shape, then the launch form, then the graph length.
+Results are cached per batch signature in a JSON file outside the repository
+(``$PYISING_AMD_TUNE_CACHE`` or ``~/.cache/pyising_amd/batch_tune.json``).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import asdict, dataclass, field
+from pathlib import Path
+
+GRAPH_CHOICES = (0, 10, 50)
+
+
+@dataclass
+class BatchTuning:
+    msc_tiles: dict = field(default_factory=dict)  # shape tuple -> "queue" | "lockstep"
+    persistent: bool = False
+    graph_sweeps: int = 50
+    flips_per_ns: float = 0.0
+    trials: list = field(default_factory=list)
+
+    def to_json(self) -> dict:
+        d = asdict(self)
+        d["msc_tiles"] = {"x".join(map(str, k)): v for k, v in self.msc_tiles.items()}
+        return d
+
+    @classmethod
+    def from_json(cls, d: dict) -> "BatchTuning":
+        tiles = {tuple(int(x) for x in k.split("x")): v for k, v in d.get("msc_tiles", {}).items()}
+        return cls(msc_tiles=tiles, persistent=bool(d.get("persistent", False)),
+                   graph_sweeps=int(d.get("graph_sweeps", 50)),
+                   flips_per_ns=float(d.get("flips_per_ns", 0.0)), trials=d.get("trials", []))
+
+
+def signature(sims) -> str:
+    """Order-independent key: the multiset of (variant, shape, algorithm)."""
+    items = sorted(f"{s.variant}:{'x'.join(map(str, s.shape))}:{s.algorithm}" for s in sims)
+    out, prev, count = [], None, 0
+    for it in items + [None]:
+        if it == prev:
+            count += 1
+            continue
+        if prev is not None:
+            out.append(f"{prev}*{count}")
+        prev, count = it, 1
+    return ";".join(out)
+
+
+def _cache_path() -> Path:
+    env = os.environ.get("PYISING_AMD_TUNE_CACHE")
+    return Path(env) if env else Path.home() / ".cache" / "pyising_amd" / "batch_tune.json"
+
+
+def _load_cache() -> dict:
+    try:
+        return json.loads(_cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_cache(cache: dict) -> None:
+    path = _cache_path()
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = path.with_suffix(".tmp")
+        tmp.write_text(json.dumps(cache, indent=1, sort_keys=True))
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home: tuning still works, it is just not remembered
+
+
+def _scratch(sims):
+    out = []
+    for i, s in enumerate(sims):
+        out.append(type(s)(s.shape, s.beta, seed=(s.seed ^ 0x9E3779B97F4A7C15) + i,
+                           algorithm=s.algorithm, init="hot"))
+    return out
+
+
+def _rate(scratch, tiles, persistent, graph_sweeps, sweeps) -> float:
+    from . import _lib
+    from .batch import BatchScan
+
+    scan = BatchScan(scratch, msc_tiles=tiles, persistent=persistent)
+    try:
+        n_spins = int(scan.n_spins.sum())
+        e0, e1 = _lib.Event(), _lib.Event()
+        if graph_sweeps and scan.capture(graph_sweeps):
+            reps = max(1, sweeps // graph_sweeps)
+            scan.replay(1)
+            scan.synchronize()
+            e0.record(scan.stream.ptr)
+            scan.replay(reps)
+            e1.record(scan.stream.ptr)
+            done = reps * graph_sweeps
+        else:
+            scan.sweep(2)
+            scan.synchronize()
+            e0.record(scan.stream.ptr)
+            scan.sweep(sweeps)
+            e1.record(scan.stream.ptr)
+            done = sweeps
+        return done * n_spins / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+    finally:
+        scan.close()
+
+
+def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
+    """Pick the msc tile per shape, the launch form and the graph length for this batch."""
+    key = signature(sims)
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        return BatchTuning.from_json(cache[key])
+    scratch = _scratch(sims)
+    msc_shapes = sorted({tuple(s.shape) for s in sims if s.variant == "msc"})
+    best = BatchTuning(msc_tiles={sh: "queue" for sh in msc_shapes})
+    trials = []
+
+    def measure(t: BatchTuning) -> float:
+        r = _rate(scratch, t.msc_tiles, t.persistent, t.graph_sweeps, sweeps)
+        trials.append({"tiles": {"x".join(map(str, k)): v for k, v in t.msc_tiles.items()},
+                       "persistent": t.persistent, "graph_sweeps": t.graph_sweeps,
+                       "flips_per_ns": round(r, 1)})
+        return r
+
+    best.flips_per_ns = measure(best)
+    for sh in msc_shapes:
+        cand = BatchTuning(dict(best.msc_tiles), best.persistent, best.graph_sweeps)
+        cand.msc_tiles[sh] = "lockstep"
+        r = measure(cand)
+        if r > best.flips_per_ns:
+            best, best.flips_per_ns = cand, r
+    cand = BatchTuning(dict(best.msc_tiles), not best.persistent, best.graph_sweeps)
+    r = measure(cand)
+    if r > best.flips_per_ns:
+        best, best.flips_per_ns = cand, r
+    for g in GRAPH_CHOICES:
+        if g == best.graph_sweeps:
+            continue
+        cand = BatchTuning(dict(best.msc_tiles), best.persistent, g)
+        r = measure(cand)
+        if r > best.flips_per_ns:
+            best, best.flips_per_ns = cand, r
+    best.trials = trials
+    if use_cache:
+        cache[key] = best.to_json()
+        _save_cache(cache)
+    del scratch
+    return best

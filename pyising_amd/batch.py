@@ -32,7 +32,11 @@ class IsingBatchLattice(C.Structure):
                 ("t0", C.c_uint32)]
 
 
-def _describe(sim: IsingSimulation) -> IsingBatchLattice:
+#: msc tile kinds of IsingBatchLattice.kind (1 = the batch's default tile)
+MSC_TILE_KIND = {"default": 1, "lockstep": 2, "queue": 3}
+
+
+def _describe(sim: IsingSimulation, msc_tile: str = "default") -> IsingBatchLattice:
     d = IsingBatchLattice()
     shape = sim.shape
     d.ndim = len(shape)
@@ -44,7 +48,7 @@ def _describe(sim: IsingSimulation) -> IsingBatchLattice:
     if isinstance(sim, HipMultiSpin):
         if sim.mode == 2:
             raise ValueError("the eager ablation is not batched")
-        d.kind, d.heatbath = 1, 0
+        d.kind, d.heatbath = MSC_TILE_KIND[msc_tile], 0
         d.c0, d.c1 = sim.w[0].ptr, sim.w[1].ptr
     elif isinstance(sim, HipInt8):
         d.kind = 0
@@ -100,12 +104,18 @@ class BatchScan:
     """
 
     def __init__(self, sims, blocks_per_cu: int | None = None, msc_mode: int = 0,
-                 msc_cap: int = 4, persistent: bool = False, stream: _lib.Stream | None = None):
+                 msc_cap: int = 4, persistent: bool = False, stream: _lib.Stream | None = None,
+                 msc_tiles: dict | None = None):
+        """``msc_tiles``: optional {shape: "queue" | "lockstep"} choosing the tile of the msc
+        lattices of that shape (others use ``msc_mode``), e.g. from :func:`pyising_amd.autotune
+        .tune_batch`."""
         self.sims = list(sims)
         if not self.sims:
             raise ValueError("empty batch")
         self._L = _lib.lib()
-        arr = (IsingBatchLattice * len(self.sims))(*[_describe(s) for s in self.sims])
+        tiles = msc_tiles or {}
+        arr = (IsingBatchLattice * len(self.sims))(
+            *[_describe(s, tiles.get(tuple(s.shape), "default")) for s in self.sims])
         h = C.c_void_p()
         _lib.check(self._L.ising_batch_create(arr, len(self.sims), int(blocks_per_cu or 0),
                                               int(msc_mode), int(msc_cap), C.byref(h)),
@@ -117,6 +127,19 @@ class BatchScan:
         self.n_spins = np.array([s.n_spins for s in self.sims], dtype=np.int64)
         self.stream = stream if stream is not None else _lib.Stream()
         self.graph_sweeps = 0
+
+    @classmethod
+    def tuned(cls, sims, sweeps: int = 50, use_cache: bool = True) -> "BatchScan":
+        """A scan with the autotuner's choices for this batch (see pyising_amd.autotune); the
+        graph, if any, is captured already (``graph_sweeps`` > 0: use :meth:`replay`)."""
+        from .autotune import tune_batch
+
+        t = tune_batch(sims, sweeps=sweeps, use_cache=use_cache)
+        scan = cls(sims, persistent=t.persistent, msc_tiles=t.msc_tiles)
+        scan.tuning = t
+        if t.graph_sweeps and not scan.capture(t.graph_sweeps):
+            scan.graph_sweeps = 0
+        return scan
 
     def info(self) -> dict:
         g, t, m = C.c_int(), C.c_int(), C.c_int()

@@ -34,12 +34,16 @@
 
 namespace cg = cooperative_groups;
 
-enum { BATCH_INT8 = 0, BATCH_MSC = 1 };
+// Lattice kinds of the C ABI: int8, msc with the batch's default tile, msc with lockstep
+// 256-word tiles, msc with queue-compaction 512-word tiles (the last two let a tuner choose the
+// tile per lattice). In a descriptor, kind is int8 or msc and msc_queue says which tile.
+enum { BATCH_INT8 = 0, BATCH_MSC = 1, BATCH_MSC_LOCKSTEP = 2, BATCH_MSC_QUEUE = 3 };
 
 struct BatchDesc {
     void* c0;
     void* c1;
     u32 kind, ndim, hb, vec;  // vec: int8 sites per thread (16, 8 or 4)
+    u32 msc_queue;            // msc: 1 = queue-compaction 512-word tiles, 0 = lockstep 256
     Int8Geom g;
     Int8Table tb;
     u32 rows, nw, n_words;
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
                     a.k0 = d.k0;
                     a.k1 = d.k1;
                     u32 calls = 0, slots = 0;
-                    if (ka.msc_mode == 0) {
+                    if (d.msc_queue) {
                         plus = batch_msc_tile(a, d.codes, local * 512u, lds);
                     } else {
                         const u32 w = local * 256u + threadIdx.x;
@@ -236,7 +240,7 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
         a.k0 = d.k0;
         a.k1 = d.k1;
         u32 calls = 0, slots = 0;
-        if (ka.msc_mode == 0) {
+        if (d.msc_queue) {
             plus = batch_msc_tile(a, d.codes, local * 512u, lds);
         } else {
             const u32 w = local * 256u + threadIdx.x;
@@ -271,7 +275,7 @@ __global__ void batch_zero_u64(unsigned long long* p, u32 n)
 // ---------------------------------------------------------------------- host side ----
 // Caller-side description of one lattice (the C ABI struct).
 typedef struct {
-    int kind;       // 0 = int8, 1 = msc
+    int kind;       // 0 = int8, 1 = msc (batch default tile), 2 = msc lockstep, 3 = msc queue
     int ndim;       // 2 or 3 (msc: 2)
     int heatbath;   // int8 only
     int lz, ly, lx; // lattice shape, lz = 1 in 2D
@@ -347,7 +351,7 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
             return ising_fail(ISING_EINVAL, "batch: bad shape");
         d.c0 = L.c0;
         d.c1 = L.c1;
-        d.kind = (u32)L.kind;
+        d.kind = (u32)(L.kind >= BATCH_MSC ? BATCH_MSC : L.kind);
         d.ndim = (u32)L.ndim;
         d.hb = (u32)(L.heatbath != 0);
         d.k0 = (u32)(L.seed & 0xFFFFFFFFull);
@@ -372,17 +376,18 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
             d.g.div_ly = make_fastdiv(d.g.ly);
             for (int k = 0; k <= 2 * L.ndim; ++k) d.tb.v[k] = L.table[k];
             d.n_tiles = (d.g.n_chunks + 255u) / 256u;
-        } else if (L.kind == BATCH_MSC) {
+        } else if (L.kind >= BATCH_MSC && L.kind <= BATCH_MSC_QUEUE) {
             if (L.ndim != 2 || L.lz != 1 || L.lx % 128) return ising_fail(ISING_EINVAL, "batch: msc needs 2D, Lx % 128 == 0");
             d.rows = (u32)L.ly;
             d.nw = (u32)(L.lx / 128);
             d.n_words = d.rows * d.nw;
             d.div_nw = make_fastdiv(d.nw);
             d.codes = msc_level_codes(L.table[3], L.table[4]);
-            const u32 per = msc_mode == 0 ? 512u : 256u;
+            d.msc_queue = L.kind == BATCH_MSC_QUEUE || (L.kind == BATCH_MSC && msc_mode == 0);
+            const u32 per = d.msc_queue ? 512u : 256u;
             d.n_tiles = (d.n_words + per - 1u) / per;
         } else {
-            return ising_fail(ISING_EINVAL, "batch: kind must be 0 (int8) or 1 (msc)");
+            return ising_fail(ISING_EINVAL, "batch: kind must be 0 (int8), 1 (msc), 2 (msc lockstep) or 3 (msc queue)");
         }
         if ((unsigned long long)tiles + d.n_tiles > 0x7FFFFFFFull) return ising_fail(ISING_ERANGE, "batch: too many tiles");
         d.tile_begin = tiles;

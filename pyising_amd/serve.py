@@ -6,10 +6,10 @@ seed, a hot start) and gets back the exact final (M, E). Design:
 
 * Memory. The server makes one device allocation for request state (``budget_bytes``) and one
   staging buffer (hot starts are generated there on the device) at start-up, and nothing after
-  that: a host-side first-fit
-  arena (:class:`DeviceArena`) hands every admitted request one region for its colour arrays and
-  reduction scratch, returned when the request finishes. Admission is by that footprint, so a
-  request is admitted exactly when its state fits the free memory and a batch slot is free.
+  that: a host-side first-fit arena (:class:`DeviceArena`) hands every admitted request one
+  region for its colour arrays and reduction scratch, returned when the request finishes.
+  Admission is by that footprint, so a request is admitted exactly when its state fits the
+  free memory and a batch slot is free.
 * Streams. Admissions (device hot start, split into colour arrays) and result reads run on a
   setup stream; batched sweeps run on a work stream. Both are non-blocking streams, and the loop
   calls no hipMalloc/hipFree, so admission work overlaps the sweeps on the device.
