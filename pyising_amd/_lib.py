@@ -241,3 +241,10 @@ class Stream:
 
 def device_sync() -> None:
     check(lib().ising_device_sync(), "device_sync")
+
+
+def mem_info() -> tuple[int, int]:
+    """(free, total) bytes of the current device."""
+    free, total = C.c_size_t(), C.c_size_t()
+    check(lib().ising_mem_info(C.byref(free), C.byref(total)), "mem_info")
+    return free.value, total.value
