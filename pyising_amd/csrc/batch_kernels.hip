@@ -365,7 +365,7 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
             const u32 lh = (u32)(L.lx / 2);
             const u32 v = (lh % 16 == 0) ? 16u : (lh % 8 == 0) ? 8u : (lh % 4 == 0) ? 4u : 0u;
             if (!v) return ising_fail(ISING_EINVAL, "batch: int8 needs Lx % 8 == 0");
-            if (rows * (lh / v) > 0xFFFFFFFFll) return ising_fail(ISING_ERANGE, "batch: int8 lattice too large");
+            if (rows * (lh / v) + (1ll << 20) > 0xFFFFFFFFll) return ising_fail(ISING_ERANGE, "batch: int8 lattice too large");
             d.vec = v;
             d.g.lh = lh;
             d.g.ly = (u32)L.ly;
@@ -378,6 +378,8 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
             d.n_tiles = (d.g.n_chunks + 255u) / 256u;
         } else if (L.kind >= BATCH_MSC && L.kind <= BATCH_MSC_QUEUE) {
             if (L.ndim != 2 || L.lz != 1 || L.lx % 128) return ising_fail(ISING_EINVAL, "batch: msc needs 2D, Lx % 128 == 0");
+            if ((long long)L.ly * (L.lx / 128) + (1ll << 20) > 0xFFFFFFFFll)
+                return ising_fail(ISING_ERANGE, "batch: msc lattice too large");
             d.rows = (u32)L.ly;
             d.nw = (u32)(L.lx / 128);
             d.n_words = d.rows * d.nw;

@@ -156,7 +156,7 @@ static int make_geom(long long rows, int lh, int ly, int lz, int ndim, Int8Geom*
     g->ly = (u32)ly;
     g->lz = (u32)lz;
     const u64 n_chunks = (u64)rows * (u64)(lh / v);
-    if (n_chunks + 256 > 0xFFFFFFFFull)
+    if (n_chunks + (1ull << 20) > 0xFFFFFFFFull)  // chunk loops step by at most 2^20
         return ising_fail(ISING_ERANGE, "int8: lattice too large (rows * Lx/2 / V must be < 2^32)");
     g->chunks_row = (u32)(lh / v);
     g->n_chunks = (u32)n_chunks;

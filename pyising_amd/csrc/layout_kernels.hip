@@ -20,6 +20,9 @@ __device__ __forceinline__ u32 spin_byte(u32 v)  // v: input byte -> 0x01 / 0xFF
     return ((int8_t)(uint8_t)v > 0) ? 0x01u : 0xFFu;
 }
 
+// Grid-stride loops throughout: a dispatch may not hold 2^32 work-items, and a large lattice
+// (msc 131072^2: 2^27 words x 64 lanes) needs more than that.
+
 // one thread per 8 full-lattice bytes (4 sites of each colour); lx % 8 == 0
 __global__ __launch_bounds__(256) void split_int8_kernel(const uint8_t* __restrict__ full,
                                                          int8_t* __restrict__ c0,
@@ -27,23 +30,25 @@ __global__ __launch_bounds__(256) void split_int8_kernel(const uint8_t* __restri
                                                          u32 lx)
 {
     const u32 per_row = lx / 8;
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (u64)rows * per_row) return;
-    const u32 r = (u32)(i / per_row);
-    const u32 k = (u32)(i - (u64)r * per_row);
-    const uint2 v = *reinterpret_cast<const uint2*>(full + (u64)r * lx + 8ull * k);
-    u32 ev = 0, od = 0;
+    const u64 n = (u64)rows * per_row;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32 r = (u32)(i / per_row);
+        const u32 k = (u32)(i - (u64)r * per_row);
+        const uint2 v = *reinterpret_cast<const uint2*>(full + (u64)r * lx + 8ull * k);
+        u32 ev = 0, od = 0;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const u32 src = b < 2 ? v.x : v.y;
-        const u32 sh = 16 * (b & 1);
-        ev |= spin_byte((src >> sh) & 0xFFu) << (8 * b);
-        od |= spin_byte((src >> (sh + 8)) & 0xFFu) << (8 * b);
+        for (int b = 0; b < 4; ++b) {
+            const u32 src = b < 2 ? v.x : v.y;
+            const u32 sh = 16 * (b & 1);
+            ev |= spin_byte((src >> sh) & 0xFFu) << (8 * b);
+            od |= spin_byte((src >> (sh + 8)) & 0xFFu) << (8 * b);
+        }
+        const u32 par = row_par(r, ly);
+        const u64 o = (u64)r * (lx / 2) + 4ull * k;
+        *reinterpret_cast<u32*>(c0 + o) = par ? od : ev;
+        *reinterpret_cast<u32*>(c1 + o) = par ? ev : od;
     }
-    const u32 par = row_par(r, ly);
-    const u64 o = (u64)r * (lx / 2) + 4ull * k;
-    *reinterpret_cast<u32*>(c0 + o) = par ? od : ev;
-    *reinterpret_cast<u32*>(c1 + o) = par ? ev : od;
 }
 
 __global__ __launch_bounds__(256) void merge_int8_kernel(const int8_t* __restrict__ c0,
@@ -52,46 +57,51 @@ __global__ __launch_bounds__(256) void merge_int8_kernel(const int8_t* __restric
                                                          u32 ly, u32 lx)
 {
     const u32 per_row = lx / 8;
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (u64)rows * per_row) return;
-    const u32 r = (u32)(i / per_row);
-    const u32 k = (u32)(i - (u64)r * per_row);
-    const u64 o = (u64)r * (lx / 2) + 4ull * k;
-    const u32 a = *reinterpret_cast<const u32*>(c0 + o);
-    const u32 b = *reinterpret_cast<const u32*>(c1 + o);
-    const u32 par = row_par(r, ly);
-    const u32 ev = par ? b : a, od = par ? a : b;
-    u32 lo = 0, hi = 0;
+    const u64 n = (u64)rows * per_row;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32 r = (u32)(i / per_row);
+        const u32 k = (u32)(i - (u64)r * per_row);
+        const u64 o = (u64)r * (lx / 2) + 4ull * k;
+        const u32 a = *reinterpret_cast<const u32*>(c0 + o);
+        const u32 b = *reinterpret_cast<const u32*>(c1 + o);
+        const u32 par = row_par(r, ly);
+        const u32 ev = par ? b : a, od = par ? a : b;
+        u32 lo = 0, hi = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const u32 pair = ((ev >> (8 * q)) & 0xFFu) | (((od >> (8 * q)) & 0xFFu) << 8);
-        if (q < 2) lo |= pair << (16 * q);
-        else hi |= pair << (16 * (q - 2));
+        for (int q = 0; q < 4; ++q) {
+            const u32 pair = ((ev >> (8 * q)) & 0xFFu) | (((od >> (8 * q)) & 0xFFu) << 8);
+            if (q < 2) lo |= pair << (16 * q);
+            else hi |= pair << (16 * (q - 2));
+        }
+        *reinterpret_cast<uint2*>(full + (u64)r * lx + 8ull * k) = make_uint2(lo, hi);
     }
-    *reinterpret_cast<uint2*>(full + (u64)r * lx + 8ull * k) = make_uint2(lo, hi);
 }
 
 // one wave per packed word pair (row r, word x): lane b handles colour site j = 64 x + b,
 // reads the two bytes at x_full = 2 j, 2 j + 1 and the wave ballot is the word. 2D: ly = rows.
+// The word index is uniform across the wave, so every wave runs the loop in lockstep.
 __global__ __launch_bounds__(256) void split_pack_kernel(const uint8_t* __restrict__ full,
                                                          u64* __restrict__ w0, u64* __restrict__ w1,
                                                          u32 rows, u32 nw)
 {
-    const u64 wi = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const u32 lane = threadIdx.x & 63u;
-    const bool ok = wi < (u64)rows * nw;
-    const u64 wc = ok ? wi : 0;
-    const u32 r = (u32)(wc / nw);
-    const u32 x = (u32)(wc - (u64)r * nw);
+    const u64 n = (u64)rows * nw;
+    const u64 stride = ((u64)gridDim.x * blockDim.x) >> 6;
     const u32 lx = nw * 128u;
-    const uint16_t pair = *reinterpret_cast<const uint16_t*>(full + (u64)r * lx + 2ull * (64ull * x + lane));
-    const bool ev = (int8_t)(pair & 0xFF) > 0;
-    const bool od = (int8_t)(pair >> 8) > 0;
-    const u64 bev = __ballot(ev), bod = __ballot(od);
-    if (ok && lane == 0) {
-        const bool par = r & 1u;
-        w0[wi] = par ? bod : bev;
-        w1[wi] = par ? bev : bod;
+    for (u64 wi = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wi < n; wi += stride) {
+        const u32 r = (u32)(wi / nw);
+        const u32 x = (u32)(wi - (u64)r * nw);
+        const uint16_t pair =
+            *reinterpret_cast<const uint16_t*>(full + (u64)r * lx + 2ull * (64ull * x + lane));
+        const bool ev = (int8_t)(pair & 0xFF) > 0;
+        const bool od = (int8_t)(pair >> 8) > 0;
+        const u64 bev = __ballot(ev), bod = __ballot(od);
+        if (lane == 0) {
+            const bool par = r & 1u;
+            w0[wi] = par ? bod : bev;
+            w1[wi] = par ? bev : bod;
+        }
     }
 }
 
@@ -100,18 +110,20 @@ __global__ __launch_bounds__(256) void unpack_merge_kernel(const u64* __restrict
                                                            uint8_t* __restrict__ full, u32 rows,
                                                            u32 nw)
 {
-    const u64 wi = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const u32 lane = threadIdx.x & 63u;
-    if (wi >= (u64)rows * nw) return;
-    const u32 r = (u32)(wi / nw);
-    const u32 x = (u32)(wi - (u64)r * nw);
-    const bool par = r & 1u;
-    const u64 a = w0[wi], b = w1[wi];
-    const u64 ev = par ? b : a, od = par ? a : b;
-    const u32 se = ((ev >> lane) & 1ull) ? 0x01u : 0xFFu;
-    const u32 so = ((od >> lane) & 1ull) ? 0x01u : 0xFFu;
-    *reinterpret_cast<uint16_t*>(full + (u64)r * (nw * 128ull) + 2ull * (64ull * x + lane)) =
-        (uint16_t)(se | (so << 8));
+    const u64 n = (u64)rows * nw;
+    const u64 stride = ((u64)gridDim.x * blockDim.x) >> 6;
+    for (u64 wi = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wi < n; wi += stride) {
+        const u32 r = (u32)(wi / nw);
+        const u32 x = (u32)(wi - (u64)r * nw);
+        const bool par = r & 1u;
+        const u64 a = w0[wi], b = w1[wi];
+        const u64 ev = par ? b : a, od = par ? a : b;
+        const u32 se = ((ev >> lane) & 1ull) ? 0x01u : 0xFFu;
+        const u32 so = ((od >> lane) & 1ull) ? 0x01u : 0xFFu;
+        *reinterpret_cast<uint16_t*>(full + (u64)r * (nw * 128ull) + 2ull * (64ull * x + lane)) =
+            (uint16_t)(se | (so << 8));
+    }
 }
 
 __global__ __launch_bounds__(256) void fill_u64_kernel(u64* __restrict__ p, u64 n, u64 v)
@@ -176,9 +188,12 @@ __global__ __launch_bounds__(256) void hot_start_kernel(u64* __restrict__ out, u
     }
 }
 
+// Blocks of 256 for `threads` work-items of a grid-stride layout kernel, at most 8192 blocks
+// (32 per CU: several generations of workgroups, enough to stream at full bandwidth).
 static unsigned grid_for(u64 threads)
 {
-    return (unsigned)((threads + 255) / 256);
+    const u64 blocks = (threads + 255) / 256;
+    return (unsigned)(blocks < 8192 ? blocks : 8192);
 }
 
 extern "C" {
@@ -245,7 +260,10 @@ int ising_hot_start(void* d_out, long long n, uint64_t state_hi, uint64_t state_
     if (n == 0) return ISING_OK;
     const u64 draws = (u64)n / 8u;
     const u64 threads = (draws + HOT_DRAWS - 1u) / HOT_DRAWS;
-    hipLaunchKernelGGL(hot_start_kernel, dim3(grid_for(threads)), dim3(256), 0, as_stream(stream),
+    // one thread per HOT_DRAWS draws, each jumping ahead once: a single generation of threads
+    if (threads > 0xFFFFFF00ull) return ising_fail(ISING_ERANGE, "hot_start: n too large");
+    hipLaunchKernelGGL(hot_start_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       as_stream(stream),
                        (u64*)d_out, draws, U128{state_hi, state_lo}, U128{inc_hi, inc_lo});
     ISING_LAUNCHED("hot_start_kernel");
     return ISING_OK;

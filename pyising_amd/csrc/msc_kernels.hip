@@ -144,7 +144,8 @@ static int msc_check(const void* own, const void* other, long long rows, int nw)
 {
     if (!own || !other) return ising_fail(ISING_EINVAL, "msc: null pointer");
     if (rows < 2 || (rows & 1) || nw < 1) return ising_fail(ISING_EINVAL, "msc: bad geometry");
-    if ((unsigned long long)rows * (unsigned long long)nw + 65536ull > 0xFFFFFFFFull)
+    // the word loops step by at most 2^20 and must not wrap a 32-bit index
+    if ((unsigned long long)rows * (unsigned long long)nw + (1ull << 20) > 0xFFFFFFFFull)
         return ising_fail(ISING_ERANGE, "msc: rows * Lx/128 must be < 2^32 words per colour");
     return ISING_OK;
 }

@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import datetime as dt
+import gc
 import json
 import math
 import platform
@@ -118,11 +119,13 @@ def max_size(cls, dim: int, frac: float = 0.8) -> int:
     free = _lib.mem_info()[0]
     per_spin = cls.bytes_per_update / 3.0 + 1.0
     L = int((frac * free / per_spin) ** (1.0 / dim))
-    m = cls.layout_multiple
+    # int8: Lx % 32 == 0 gives 16 sites per lane; the kernels' 32-bit chunk index (sites / 32)
+    # or word index (spins / 128 per colour) must stay 2^20 below 2^32
+    m = cls.layout_multiple if cls.stream == "msc" else 32
+    per_index = 128 if cls.stream == "msc" else 32
     L = max(m, (L // m) * m)
-    if cls.stream == "int8":  # kernel index limit: rows * Lx/2 / 16 chunks < 2^32
-        while L**dim // 32 + 256 >= 1 << 32:
-            L -= m
+    while L**dim // per_index + (1 << 20) > (1 << 32) - 1:
+        L -= m
     return L
 
 
@@ -227,7 +230,10 @@ def run(mode: str = "quick", out: str | Path | None = None, repeats: int | None 
                 r = time_one(cls, shape, algorithm, beta, repeats, min_time)
             except (RuntimeError, MemoryError, ValueError) as exc:  # e.g. out of memory
                 log(f"FAIL hip/{variant} {shape}: {type(exc).__name__}: {exc}"[:300])
+                gc.collect()
                 continue
+            finally:
+                gc.collect()  # release the lattice's device memory before the next size
             rec = run_record(cls, variant, algorithm, shape, r, ceiling, repeats, beta)
             runs.append(rec)
             log(f"{'hip':>6}/{variant:<12} {algorithm[:5]} {str(shape):>22}: "

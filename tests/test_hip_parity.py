@@ -104,13 +104,41 @@ def test_device_hot_start_equals_numpy_draws(variant, shape, seed):
     np.testing.assert_array_equal(sim.get_lattice(), O.random_lattice(shape, seed))
 
 
+@pytest.mark.parametrize("variant,shape", [("int8", (4096, 8192)), ("msc", (1024, 8192))])
+def test_device_hot_start_large(variant, shape):
+    """Hot starts larger than one generation of layout workgroups."""
+    from pyising_amd import create
+
+    sim = create("hip", variant, shape=shape, beta=0.4, seed=3, init="hot")
+    np.testing.assert_array_equal(sim.get_lattice(), O.random_lattice(shape, 3))
+
+
+def test_msc_131072_set_up_covers_every_word(hip):
+    """A 131072^2 msc lattice (2^27 words per colour: more work-items than one dispatch may
+    hold) is set up whole: the cold start gives M = N, E = -2N exactly, and the hot start's M
+    and E are within 6 sigma of 0 (independent fair spins: var M = N, var E = 2N)."""
+    shape = (131072, 131072)
+    n = shape[0] * shape[1]
+    cold = hip.HipMultiSpin(shape, 1 / 2.269, seed=1, init="cold")
+    assert cold.observables_exact() == (n, -2 * n)
+    del cold
+    hot = hip.HipMultiSpin(shape, 1 / 2.269, seed=1, init="hot")
+    m, e = hot.observables_exact()
+    assert abs(m) < 6 * n**0.5 and abs(e) < 6 * (2 * n) ** 0.5, (m, e)
+    hot.sweep(1)
+    m1, e1 = hot.observables_exact()
+    assert (m1, e1) != (m, e) and e1 < -0.5 * n  # one sweep at T_c orders most bonds
+
+
 def test_cold_start_3d(hip):
     sim = hip.HipInt8((8, 8, 16), 0.3, init="cold")
     assert sim.observables_exact() == (1024, -3 * 1024)
 
 
+# the last three exceed the layout kernels' 8192-block grid, so their grid-stride loops wrap
 @pytest.mark.parametrize("variant,shape", [("int8", (16, 24)), ("int8", (6, 4, 8)),
-                                           ("msc", (64, 384))])
+                                           ("msc", (64, 384)), ("int8", (4096, 8192)),
+                                           ("int8", (64, 64, 8192)), ("msc", (1024, 8192))])
 def test_layout_roundtrip(hip, variant, shape):
     cls = hip.HipInt8 if variant == "int8" else hip.HipMultiSpin
     lat = O.random_lattice(shape, 123)
