@@ -208,8 +208,7 @@ def main() -> int:
         "data": "synthetic (hot start: numpy default_rng(1) draws)",
         "config": {"workload": "2D multi-spin-coded Metropolis sweep, L=16384 torus, "
                                "beta=1/T_c, seed=1, hot start; one step = one sweep",
-                   "L": L_HEADLINE, "beta": BETA_C_2D, "seed": 1, "kernel_mode": sim.mode,
-                   "tile_k": sim.tile_k},
+                   "L": L_HEADLINE, "beta": BETA_C_2D, "seed": 1, "kernel": sim.kernel()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profile(),

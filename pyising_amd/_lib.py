@@ -56,6 +56,7 @@ SIGNATURES: dict[str, list] = {
     "ising_int8_sweeps": [_P, _P, _LL, _I, _I, _I, _I, _I, _P, _U32, _LL, _U32, _U32, _P],
     "ising_int8_obs": [_P, _P, _LL, _I, _I, _I, _I, _P, _P, _P],
     "ising_msc_config": [_I, _I],
+    "ising_msc_kernel": [_LL, _I, _I, _I, _P, _P],
     "ising_msc_half": [_P, _P, _LL, _I, _U32, _U32, _U32, _U32, _U32, _U32, _P, _P],
     "ising_msc_sweeps": [_P, _P, _LL, _I, _U32, _LL, _U32, _U32, _U32, _U32, _P, _P],
     "ising_msc_half_ex": [_P, _P, _LL, _I, _U32, _U32, _U32, _U32, _U32, _U32, _I, _I, _P, _P],

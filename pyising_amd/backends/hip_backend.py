@@ -13,6 +13,8 @@ exact integer reductions on the device.
 
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 
 from .. import _lib
@@ -23,7 +25,7 @@ _ONES_INT8 = 0x0101010101010101
 _ALL_ONES = 0xFFFFFFFFFFFFFFFF
 
 #: msc kernel modes of ising_msc_sweeps_ex
-MSC_COMPACT, MSC_LOCKSTEP, MSC_EAGER = 0, 1, 2
+MSC_DEFAULT, MSC_COMPACT, MSC_LOCKSTEP, MSC_EAGER, MSC_BY_SIZE = -1, 0, 1, 2, 3
 
 
 def _full_lattice(lat: np.ndarray, shape) -> np.ndarray:
@@ -183,8 +185,9 @@ class HipMultiSpin(_HipBase):
     layout_multiple = 128
     stream = "msc"
     bytes_per_update = 3.0 / 8.0
-    #: kernel mode and its parameter (compaction: 10 * cap + R, see ising_msc_config)
-    mode = MSC_COMPACT
+    #: kernel mode and its parameter (compaction: 10 * cap + R, see ising_msc_config); the
+    #: process default picks the lockstep tile up to 8192^2 and queue compaction above
+    mode = MSC_DEFAULT
     tile_k = 42
 
     def _setup(self) -> None:
@@ -221,6 +224,14 @@ class HipMultiSpin(_HipBase):
         if tmp:
             buf.free()
         return full
+
+    def kernel(self) -> dict:
+        """The kernel a half sweep of this lattice runs: {"mode": 0 | 1 | 2, "name", "k"}."""
+        mode, k = C.c_int(), C.c_int()
+        _lib.check(self._L.ising_msc_kernel(self.rows, self.nw, self.mode, self.tile_k,
+                                            C.byref(mode), C.byref(k)), "msc_kernel")
+        names = {MSC_COMPACT: "queue compaction", MSC_LOCKSTEP: "lockstep", MSC_EAGER: "eager"}
+        return {"mode": mode.value, "name": names[mode.value], "k": k.value}
 
     def get_words(self) -> tuple[np.ndarray, np.ndarray]:
         """The two packed colour arrays, (rows, Lx/128) uint64 each."""

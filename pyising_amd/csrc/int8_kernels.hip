@@ -135,12 +135,20 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n)
 }
 
 // ---------------------------------------------------------------------- host launchers ----
-static int pick_vec(u32 lh)
+// Sites per lane: the widest of 16, 8, 4 dividing lh that still gives >= 2^17 lanes (512
+// workgroups, two per CU), else the narrowest that divides lh. A half sweep of a small lattice
+// is one generation of workgroups and takes a lane's latency, which grows with the sites it
+// updates; large lattices stream at full bandwidth with the widest vectors. V only tiles the
+// sites: the Philox counter of a site is its quad index, so every V gives the same spins.
+static int pick_vec(u32 lh, long long rows)
 {
-    if (lh % 16 == 0) return 16;
-    if (lh % 8 == 0) return 8;
-    if (lh % 4 == 0) return 4;
-    return 0;
+    int narrow = 0;
+    for (int v = 16; v >= 4; v /= 2) {
+        if (lh % (u32)v) continue;
+        if ((unsigned long long)rows * (lh / (u32)v) >= (1ull << 17)) return v;
+        narrow = v;
+    }
+    return narrow;
 }
 
 static int make_geom(long long rows, int lh, int ly, int lz, int ndim, Int8Geom* g, int* V)
@@ -149,7 +157,7 @@ static int make_geom(long long rows, int lh, int ly, int lz, int ndim, Int8Geom*
         return ising_fail(ISING_EINVAL, "int8: bad geometry");
     if ((long long)ly * lz != rows) return ising_fail(ISING_EINVAL, "int8: rows != ly * lz");
     if (ndim == 2 && lz != 1) return ising_fail(ISING_EINVAL, "int8: 2D requires lz == 1");
-    const int v = pick_vec((u32)lh);
+    const int v = pick_vec((u32)lh, rows);
     if (!v) return ising_fail(ISING_EINVAL, "int8: lh must be a multiple of 4 (Lx % 8 == 0)");
     if (ly < 2 || lh < 4 || (ndim == 3 && lz < 2)) return ising_fail(ISING_EINVAL, "int8: extents too small");
     g->lh = (u32)lh;

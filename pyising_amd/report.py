@@ -86,14 +86,14 @@ def msc_calls_per_word(L: int = 4096, mode: int | None = None, sweeps: int = 20)
 
 def measure_ceilings(quick: bool = False) -> dict:
     """Copy bandwidth, Philox rate and msc Philox work (bench/roofline.py:97-117 keys)."""
-    from .backends.hip_backend import MSC_LOCKSTEP
+    from .backends.hip_backend import MSC_COMPACT, MSC_LOCKSTEP
     from .probes import copy_bandwidth, philox_rate
 
     nbytes = (1 << 28) if quick else (1 << 32)
     bw = copy_bandwidth(nbytes, repeats=10 if quick else 20)
     ph = philox_rate(128 if quick else 512)
     L = 2048 if quick else 4096
-    queue = msc_calls_per_word(L)
+    queue = msc_calls_per_word(L, MSC_COMPACT)
     lockstep = msc_calls_per_word(L, MSC_LOCKSTEP)
     return {
         "copy_bandwidth_GBs": round(bw, 1),

@@ -66,9 +66,33 @@ def test_argument_validation_needs_no_gpu():
     for k in (43, 72, 22, 41, 151):
         rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 0, k, None, None)
         assert rc == -1 and b"cap" in lib.ising_last_error()
-    rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 3, 42, None, None)
+    rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 4, 42, None, None)
     assert rc == -1 and b"mode" in lib.ising_last_error()
     rc = lib.ising_msc_half_ex(1, 1, 15, 1, 0, 0, 0, 0, 0, 0, 0, 42, None, None)
     assert rc == -1 and b"geometry" in lib.ising_last_error()
     rc = lib.ising_split_pack(1, 1, 1, 4, 100, None)
     assert rc == -1 and b"128" in lib.ising_last_error()
+
+
+def test_msc_kernel_by_size_needs_no_gpu():
+    """The default msc kernel is the lockstep tile up to 2^19 words per colour (8192^2) and
+    queue compaction above; explicit modes pass through."""
+    import ctypes as C
+
+    lib = _lib.load_library()
+    mode, k = C.c_int(), C.c_int()
+
+    def kernel(rows, nw, m=-1, kk=42):
+        assert lib.ising_msc_kernel(rows, nw, m, kk, C.byref(mode), C.byref(k)) == 0
+        return mode.value, k.value
+
+    assert kernel(1024, 8) == (1, 0)
+    assert kernel(8192, 64) == (1, 0)  # 2^19 words
+    assert kernel(8194, 64) == (0, 42)
+    assert kernel(16384, 128) == (0, 42)
+    assert kernel(16384, 128, 3, 44) == (0, 44)
+    assert kernel(1024, 8, 0, 32) == (0, 32)
+    assert kernel(16384, 128, 1, 0) == (1, 0)
+    assert kernel(16384, 128, 2, 0) == (2, 0)
+    assert lib.ising_msc_kernel(1024, 8, 5, 42, C.byref(mode), C.byref(k)) == -1
+    assert lib.ising_msc_kernel(1024, 8, 3, 41, C.byref(mode), C.byref(k)) == -1

@@ -226,6 +226,21 @@ def test_config_int8_L2048_temperatures(hip, T):
     np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
 
 
+# sites per lane follow the lattice size (the widest of 16/8/4 dividing Lx/2 that gives >= 2^17
+# lanes): (2048, 1024) and (64, 128, 256) take 8, (128, 128, 256) takes 16, (32, 32, 32) and
+# (1024, 1024) take 4, (2048, 2048) takes 16; every width must give the same spins
+@pytest.mark.parametrize("shape", [(2048, 1024), (64, 128, 256), (128, 128, 256)])
+@pytest.mark.parametrize("algorithm", ["metropolis", "heatbath"])
+def test_int8_every_vector_width(hip, shape, algorithm):
+    beta = 0.44 if len(shape) == 2 else 0.2216
+    sim = hip.HipInt8(shape, beta, seed=99, algorithm=algorithm)
+    ref = CO.CInt8(shape, beta, seed=99, algorithm=algorithm)
+    sim.sweep(3)
+    ref.sweep(3)
+    assert sim.observables_exact() == ref.observables_exact()
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+
+
 def test_config_msc_L16384_tc_few_sweeps(hip):
     """The headline lattice itself: msc L=16384 at T_c, seed=1, hot start, 3 sweeps."""
     from pyising_amd import BETA_C_2D
