@@ -108,6 +108,7 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
             "batched_flips_per_ns": round(batch_rate, 1), "plain_flips_per_ns": round(plain_rate, 1),
             "speedup": round(batch_rate / plain_rate, 2),
             "tuning": {"msc_tiles": {"x".join(map(str, k)): v for k, v in tuning.msc_tiles.items()},
+                       "int8_vecs": {"x".join(map(str, k)): v for k, v in tuning.int8_vecs.items()},
                        "persistent": tuning.persistent, "graph_sweeps": tuning.graph_sweeps},
             "binder_L4096_Tmid": round(mom[-9].binder, 4)}
 

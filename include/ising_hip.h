@@ -170,6 +170,9 @@ typedef struct {
     void* c0;            /* device colour arrays (int8 (rows, lx/2) or uint64 (rows, lx/128)) */
     void* c1;
     uint32_t t0;         /* sweep counter at batch start */
+    int vec;             /* int8: sites per lane, 16, 8 or 4 dividing lx/2; 0 = the widest that
+                          * still fills one 256-lane tile (else the narrowest). Any width gives
+                          * the same spins. */
 } IsingBatchLattice;
 /* blocks_per_cu <= 0: occupancy maximum; msc_mode 0 = queue-compaction tiles (512 words),
  * 1 = lockstep tiles (256 words); msc_cap = unrolled calls of the queue tiles (must be 4). */

@@ -2,11 +2,13 @@
 
 A batch's speed depends on choices that never change its results: the tile of each msc
 lattice size (queue compaction over 512 words, or lockstep over 256 words; small lattices
-have too few words to fill waves after compaction), the launch form (one launch per half
-sweep, or one persistent cooperative launch), and the number of sweeps per captured graph.
-:func:`tune_batch` times candidates on scratch lattices of the same shapes, temperatures and
-algorithms (hot starts from other seeds, so the callers' lattices are untouched), by
-coordinate descent: the msc tile per shape, then the launch form, then the graph length.
+have too few words to fill waves after compaction), the sites per lane of each int8 lattice
+size (16, 8 or 4: wide vectors amortise per-lane work, narrow ones fill the lanes of a small
+lattice's tile), the launch form (one launch per half sweep, or one persistent cooperative
+launch), and the number of sweeps per captured graph. :func:`tune_batch` times candidates on
+scratch lattices of the same shapes, temperatures and algorithms (hot starts from other seeds,
+so the callers' lattices are untouched), by coordinate descent: the msc tile per shape, the
+int8 width per shape, then the launch form, then the graph length.
 Results are cached per batch signature in a JSON file outside the repository
 (``$PYISING_AMD_TUNE_CACHE`` or ``~/.cache/pyising_amd/batch_tune.json``).
 """
@@ -24,6 +26,7 @@ GRAPH_CHOICES = (0, 10, 50)
 @dataclass
 class BatchTuning:
     msc_tiles: dict = field(default_factory=dict)  # shape tuple -> "queue" | "lockstep"
+    int8_vecs: dict = field(default_factory=dict)  # shape tuple -> 16 | 8 | 4 (absent: default)
     persistent: bool = False
     graph_sweeps: int = 50
     flips_per_ns: float = 0.0
@@ -32,12 +35,15 @@ class BatchTuning:
     def to_json(self) -> dict:
         d = asdict(self)
         d["msc_tiles"] = {"x".join(map(str, k)): v for k, v in self.msc_tiles.items()}
+        d["int8_vecs"] = {"x".join(map(str, k)): v for k, v in self.int8_vecs.items()}
         return d
 
     @classmethod
     def from_json(cls, d: dict) -> "BatchTuning":
         tiles = {tuple(int(x) for x in k.split("x")): v for k, v in d.get("msc_tiles", {}).items()}
-        return cls(msc_tiles=tiles, persistent=bool(d.get("persistent", False)),
+        vecs = {tuple(int(x) for x in k.split("x")): int(v)
+                for k, v in d.get("int8_vecs", {}).items()}
+        return cls(msc_tiles=tiles, int8_vecs=vecs, persistent=bool(d.get("persistent", False)),
                    graph_sweeps=int(d.get("graph_sweeps", 50)),
                    flips_per_ns=float(d.get("flips_per_ns", 0.0)), trials=d.get("trials", []))
 
@@ -87,11 +93,13 @@ def _scratch(sims):
     return out
 
 
-def _rate(scratch, tiles, persistent, graph_sweeps, sweeps) -> float:
+def _rate(scratch, t: BatchTuning, sweeps: int) -> float:
     from . import _lib
     from .batch import BatchScan
 
-    scan = BatchScan(scratch, msc_tiles=tiles, persistent=persistent)
+    graph_sweeps = t.graph_sweeps
+    scan = BatchScan(scratch, msc_tiles=t.msc_tiles, int8_vecs=t.int8_vecs,
+                     persistent=t.persistent)
     try:
         n_spins = int(scan.n_spins.sum())
         e0, e1 = _lib.Event(), _lib.Event()
@@ -115,42 +123,57 @@ def _rate(scratch, tiles, persistent, graph_sweeps, sweeps) -> float:
         scan.close()
 
 
+def _vec_choices(shape) -> list[int]:
+    lh = shape[-1] // 2
+    return [v for v in (16, 8, 4) if lh % v == 0]
+
+
 def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
-    """Pick the msc tile per shape, the launch form and the graph length for this batch."""
+    """Pick the msc tile and int8 width per shape, the launch form and the graph length."""
     key = signature(sims)
     cache = _load_cache() if use_cache else {}
     if key in cache:
         return BatchTuning.from_json(cache[key])
     scratch = _scratch(sims)
     msc_shapes = sorted({tuple(s.shape) for s in sims if s.variant == "msc"})
+    int8_shapes = sorted({tuple(s.shape) for s in sims if s.variant == "int8"})
     best = BatchTuning(msc_tiles={sh: "queue" for sh in msc_shapes})
     trials = []
 
+    def copy(**kw) -> BatchTuning:
+        base = {"msc_tiles": dict(best.msc_tiles), "int8_vecs": dict(best.int8_vecs),
+                "persistent": best.persistent, "graph_sweeps": best.graph_sweeps}
+        base.update(kw)
+        return BatchTuning(**base)
+
     def measure(t: BatchTuning) -> float:
-        r = _rate(scratch, t.msc_tiles, t.persistent, t.graph_sweeps, sweeps)
+        r = _rate(scratch, t, sweeps)
         trials.append({"tiles": {"x".join(map(str, k)): v for k, v in t.msc_tiles.items()},
+                       "int8_vecs": {"x".join(map(str, k)): v for k, v in t.int8_vecs.items()},
                        "persistent": t.persistent, "graph_sweeps": t.graph_sweeps,
                        "flips_per_ns": round(r, 1)})
         return r
 
+    def consider(cand: BatchTuning) -> None:
+        nonlocal best
+        r = measure(cand)
+        if r > best.flips_per_ns:
+            best, best.flips_per_ns = cand, r
+
     best.flips_per_ns = measure(best)
     for sh in msc_shapes:
-        cand = BatchTuning(dict(best.msc_tiles), best.persistent, best.graph_sweeps)
-        cand.msc_tiles[sh] = "lockstep"
-        r = measure(cand)
-        if r > best.flips_per_ns:
-            best, best.flips_per_ns = cand, r
-    cand = BatchTuning(dict(best.msc_tiles), not best.persistent, best.graph_sweeps)
-    r = measure(cand)
-    if r > best.flips_per_ns:
-        best, best.flips_per_ns = cand, r
+        tiles = dict(best.msc_tiles)
+        tiles[sh] = "lockstep"
+        consider(copy(msc_tiles=tiles))
+    for sh in int8_shapes:
+        for v in _vec_choices(sh):
+            vecs = dict(best.int8_vecs)
+            vecs[sh] = v
+            consider(copy(int8_vecs=vecs))
+    consider(copy(persistent=not best.persistent))
     for g in GRAPH_CHOICES:
-        if g == best.graph_sweeps:
-            continue
-        cand = BatchTuning(dict(best.msc_tiles), best.persistent, g)
-        r = measure(cand)
-        if r > best.flips_per_ns:
-            best, best.flips_per_ns = cand, r
+        if g != best.graph_sweeps:
+            consider(copy(graph_sweeps=g))
     best.trials = trials
     if use_cache:
         cache[key] = best.to_json()

@@ -29,14 +29,14 @@ class IsingBatchLattice(C.Structure):
     _fields_ = [("kind", C.c_int), ("ndim", C.c_int), ("heatbath", C.c_int), ("lz", C.c_int),
                 ("ly", C.c_int), ("lx", C.c_int), ("seed", C.c_ulonglong),
                 ("table", C.c_uint32 * 7), ("c0", C.c_void_p), ("c1", C.c_void_p),
-                ("t0", C.c_uint32)]
+                ("t0", C.c_uint32), ("vec", C.c_int)]
 
 
 #: msc tile kinds of IsingBatchLattice.kind (1 = the batch's default tile)
 MSC_TILE_KIND = {"default": 1, "lockstep": 2, "queue": 3}
 
 
-def _describe(sim: IsingSimulation, msc_tile: str = "default") -> IsingBatchLattice:
+def _describe(sim: IsingSimulation, msc_tile: str = "default", vec: int = 0) -> IsingBatchLattice:
     d = IsingBatchLattice()
     shape = sim.shape
     d.ndim = len(shape)
@@ -52,6 +52,7 @@ def _describe(sim: IsingSimulation, msc_tile: str = "default") -> IsingBatchLatt
         d.c0, d.c1 = sim.w[0].ptr, sim.w[1].ptr
     elif isinstance(sim, HipInt8):
         d.kind = 0
+        d.vec = int(vec)
         d.heatbath = int(sim.algorithm == "heatbath")
         d.c0, d.c1 = sim.c[0].ptr, sim.c[1].ptr
     else:
@@ -105,17 +106,19 @@ class BatchScan:
 
     def __init__(self, sims, blocks_per_cu: int | None = None, msc_mode: int = 0,
                  msc_cap: int = 4, persistent: bool = False, stream: _lib.Stream | None = None,
-                 msc_tiles: dict | None = None):
+                 msc_tiles: dict | None = None, int8_vecs: dict | None = None):
         """``msc_tiles``: optional {shape: "queue" | "lockstep"} choosing the tile of the msc
-        lattices of that shape (others use ``msc_mode``), e.g. from :func:`pyising_amd.autotune
-        .tune_batch`."""
+        lattices of that shape (others use ``msc_mode``); ``int8_vecs``: optional {shape: 16 |
+        8 | 4} sites per lane of the int8 lattices of that shape (others: the widest width that
+        fills a tile); e.g. from :func:`pyising_amd.autotune.tune_batch`."""
         self.sims = list(sims)
         if not self.sims:
             raise ValueError("empty batch")
         self._L = _lib.lib()
-        tiles = msc_tiles or {}
+        tiles, vecs = msc_tiles or {}, int8_vecs or {}
         arr = (IsingBatchLattice * len(self.sims))(
-            *[_describe(s, tiles.get(tuple(s.shape), "default")) for s in self.sims])
+            *[_describe(s, tiles.get(tuple(s.shape), "default"), vecs.get(tuple(s.shape), 0))
+              for s in self.sims])
         h = C.c_void_p()
         _lib.check(self._L.ising_batch_create(arr, len(self.sims), int(blocks_per_cu or 0),
                                               int(msc_mode), int(msc_cap), C.byref(h)),
@@ -135,7 +138,7 @@ class BatchScan:
         from .autotune import tune_batch
 
         t = tune_batch(sims, sweeps=sweeps, use_cache=use_cache)
-        scan = cls(sims, persistent=t.persistent, msc_tiles=t.msc_tiles)
+        scan = cls(sims, persistent=t.persistent, msc_tiles=t.msc_tiles, int8_vecs=t.int8_vecs)
         scan.tuning = t
         if t.graph_sweeps and not scan.capture(t.graph_sweeps):
             scan.graph_sweeps = 0

@@ -14,6 +14,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
+INCLUDE = HERE.parent / "include"
 OUT = HERE / "libising_hip.so"
 ARCH = os.environ.get("ISING_OFFLOAD_ARCH", "gfx950")
 
@@ -32,7 +33,7 @@ def sources() -> list[Path]:
 def command(out: Path = OUT) -> list[str]:
     return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-Wall", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            f"-I{CSRC}", "-o", str(out)] + [str(s) for s in sources()]
+            f"-I{CSRC}", f"-I{INCLUDE}", "-o", str(out)] + [str(s) for s in sources()]
 
 
 def build(verbose: bool = False) -> Path:

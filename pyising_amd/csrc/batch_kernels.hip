@@ -273,18 +273,8 @@ __global__ void batch_zero_u64(unsigned long long* p, u32 n)
 }
 
 // ---------------------------------------------------------------------- host side ----
-// Caller-side description of one lattice (the C ABI struct).
-typedef struct {
-    int kind;       // 0 = int8, 1 = msc (batch default tile), 2 = msc lockstep, 3 = msc queue
-    int ndim;       // 2 or 3 (msc: 2)
-    int heatbath;   // int8 only
-    int lz, ly, lx; // lattice shape, lz = 1 in 2D
-    unsigned long long seed;
-    uint32_t table[7];  // thresholds, 2 * ndim + 1 used (msc: table[3], table[4])
-    void* c0;       // device colour arrays, caller-owned
-    void* c1;
-    uint32_t t0;    // sweep counter of the lattice when the batch starts
-} IsingBatchLattice;
+// IsingBatchLattice, the caller-side description of one lattice, is the C ABI struct of
+// ising_hip.h.
 
 struct BatchHandle {
     u32 n_lat = 0, total_tiles = 0;
@@ -363,8 +353,20 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
             if (L.ndim != 2 && L.ndim != 3) return ising_fail(ISING_EINVAL, "batch: int8 ndim 2 or 3");
             if (L.ndim == 2 && L.lz != 1) return ising_fail(ISING_EINVAL, "batch: 2D needs lz = 1");
             const u32 lh = (u32)(L.lx / 2);
-            const u32 v = (lh % 16 == 0) ? 16u : (lh % 8 == 0) ? 8u : (lh % 4 == 0) ? 4u : 0u;
-            if (!v) return ising_fail(ISING_EINVAL, "batch: int8 needs Lx % 8 == 0");
+            u32 v = 0;
+            if (L.vec) {
+                if ((L.vec != 16 && L.vec != 8 && L.vec != 4) || L.lx % 8 || lh % (u32)L.vec)
+                    return ising_fail(ISING_EINVAL, "batch: int8 vec must be 16, 8 or 4 and divide Lx/2");
+                v = (u32)L.vec;
+            } else {
+                // the widest width that still fills a 256-lane tile, else the narrowest
+                for (u32 c = 16; c >= 4; c /= 2) {
+                    if (lh % c) continue;
+                    v = c;
+                    if (rows * (long long)(lh / c) >= 256) break;
+                }
+            }
+            if (!v || L.lx % 8) return ising_fail(ISING_EINVAL, "batch: int8 needs Lx % 8 == 0");
             if (rows * (lh / v) + (1ll << 20) > 0xFFFFFFFFll) return ising_fail(ISING_ERANGE, "batch: int8 lattice too large");
             d.vec = v;
             d.g.lh = lh;

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ising_hip.h"  // the C ABI: every entry point here is checked against its declaration
+
 #include <string>
 
 typedef uint32_t u32;

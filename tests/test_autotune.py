@@ -25,7 +25,8 @@ def test_signature_is_order_independent_and_counts():
 def test_tuning_json_round_trip(tmp_path, monkeypatch):
     from pyising_amd import autotune
 
-    t = BatchTuning(msc_tiles={(1024, 1024): "lockstep", (128, 256): "queue"}, persistent=True,
+    t = BatchTuning(msc_tiles={(1024, 1024): "lockstep", (128, 256): "queue"},
+                    int8_vecs={(64, 64): 8, (8, 8, 16): 4}, persistent=True,
                     graph_sweeps=10, flips_per_ns=123.4)
     assert BatchTuning.from_json(t.to_json()) == t
     monkeypatch.setenv("PYISING_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
@@ -67,6 +68,42 @@ def test_per_shape_msc_tiles_are_exact(hip, tiles):
         ref.sweep(6)
         np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
     scan.close()
+
+
+def test_vec_choices():
+    from pyising_amd.autotune import _vec_choices
+
+    assert _vec_choices((64, 64)) == [16, 8, 4]
+    assert _vec_choices((8, 8, 16)) == [8, 4]
+    assert _vec_choices((16, 24)) == [4]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("vecs", [{(64, 64): 4, (8, 8, 16): 8}, {(64, 64): 8, (8, 8, 16): 4},
+                                  {(64, 64): 16}])
+def test_per_shape_int8_widths_are_exact(hip, vecs):
+    from pyising_amd.batch import BatchScan
+
+    specs = _mixed(hip)
+    sims = [cls(sh, b, seed=sd, algorithm=al, init=lat) for cls, sh, b, al, sd, lat in specs]
+    refs = [cls(sh, b, seed=sd, algorithm=al, init=lat) for cls, sh, b, al, sd, lat in specs]
+    scan = BatchScan(sims, int8_vecs=vecs)
+    scan.sweep(6)
+    scan.synchronize()
+    for sim, ref in zip(sims, refs, strict=True):
+        ref.sweep(6)
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    scan.close()
+
+
+@pytest.mark.gpu
+def test_bad_int8_width_is_rejected(hip):
+    from pyising_amd import _lib
+    from pyising_amd.batch import BatchScan
+
+    sims = [hip.HipInt8((16, 24), 0.4, seed=1)]  # Lx/2 = 12: only 4 sites per lane divide it
+    with pytest.raises(_lib.HipError, match="vec"):
+        BatchScan(sims, int8_vecs={(16, 24): 8})
 
 
 @pytest.mark.gpu
