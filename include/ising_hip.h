@@ -189,9 +189,13 @@ int ising_batch_replay(void* handle, int replays, void* stream);
 int ising_batch_stats(void* handle, long long* out, long long* sweeps_done, void* stream);
 int ising_batch_reset_stats(void* handle, void* stream);
 int ising_batch_info(void* handle, int* grid, int* total_tiles, int* max_blocks_per_cu);
-/* Form of a sweep call: 0 (default) = one launch per half sweep + one accumulation launch per
- * sweep; 1 = one persistent cooperative launch with a grid barrier between half sweeps. */
+/* Form of a sweep call: 0 (default) = one launch per half sweep (+ one accumulation launch per
+ * sweep while moments are on); 1 = one persistent cooperative launch with a grid barrier
+ * between half sweeps. */
 int ising_batch_set_persistent(void* handle, int persistent);
+/* 1 (default): accumulate the moments of M after every sweep (one more launch per sweep in the
+ * launch form); 0: skip it, e.g. in a serving loop that reads each lattice's own (M, E). */
+int ising_batch_set_moments(void* handle, int on);
 int ising_batch_destroy(void* handle);
 /* Tiles of one half sweep of these lattices (host only, no device call): the tile capacity a
  * reserved batch needs for them. */

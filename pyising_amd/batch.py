@@ -106,11 +106,14 @@ class BatchScan:
 
     def __init__(self, sims, blocks_per_cu: int | None = None, msc_mode: int = 0,
                  msc_cap: int = 4, persistent: bool = False, stream: _lib.Stream | None = None,
-                 msc_tiles: dict | None = None, int8_vecs: dict | None = None):
+                 msc_tiles: dict | None = None, int8_vecs: dict | None = None,
+                 moments: bool = True):
         """``msc_tiles``: optional {shape: "queue" | "lockstep"} choosing the tile of the msc
         lattices of that shape (others use ``msc_mode``); ``int8_vecs``: optional {shape: 16 |
         8 | 4} sites per lane of the int8 lattices of that shape (others: the widest width that
-        fills a tile); e.g. from :func:`pyising_amd.autotune.tune_batch`."""
+        fills a tile); e.g. from :func:`pyising_amd.autotune.tune_batch`. ``moments``: False
+        skips the per-sweep accumulation of M moments (one launch per sweep fewer; moments()
+        then reports no samples)."""
         self.sims = list(sims)
         if not self.sims:
             raise ValueError("empty batch")
@@ -126,6 +129,8 @@ class BatchScan:
         self._h = h.value
         _lib.check(self._L.ising_batch_set_persistent(self._h, int(bool(persistent))),
                    "batch_set_persistent")
+        _lib.check(self._L.ising_batch_set_moments(self._h, int(bool(moments))),
+                   "batch_set_moments")
         self.persistent = bool(persistent)
         self.n_spins = np.array([s.n_spins for s in self.sims], dtype=np.int64)
         self.stream = stream if stream is not None else _lib.Stream()
@@ -229,7 +234,9 @@ class ReservedBatch:
     """
 
     def __init__(self, max_lattices: int, max_tiles: int, stream: _lib.Stream, msc_mode: int = 0,
-                 msc_cap: int = 4, blocks_per_cu: int | None = None):
+                 msc_cap: int = 4, blocks_per_cu: int | None = None, moments: bool = False):
+        """``moments``: accumulate M moments after every sweep (off by default: a serving loop
+        reads each lattice's own exact (M, E) when it finishes)."""
         self._L = _lib.lib()
         self.stream = stream
         self.msc_mode = int(msc_mode)
@@ -238,6 +245,8 @@ class ReservedBatch:
                                                int(blocks_per_cu or 0), self.msc_mode,
                                                int(msc_cap), C.byref(h)), "batch_reserve")
         self._h = h.value
+        _lib.check(self._L.ising_batch_set_moments(self._h, int(bool(moments))),
+                   "batch_set_moments")
         self.max_lattices, self.max_tiles = int(max_lattices), int(max_tiles)
         self.sims: list[IsingSimulation] = []
 

@@ -64,6 +64,9 @@ struct BatchKernelArgs {
     const u32* tile_lat;
     u32* plus;  // [2 sweep parity][2 colour][total_tiles]: +1 spins written by each tile
     BatchAcc* acc;
+    // [0]: sweeps done, read by colour-0 work; [1]: the sweep in progress, written by colour-0
+    // launches and read by colour-1 launches (each launch only writes the counter the
+    // concurrent workgroups of the same launch do not read)
     unsigned long long* sweeps_done;
     u32 n_lat, total_tiles, n_sweeps, msc_mode, msc_cap;
 };
@@ -209,13 +212,19 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
 
 // ------------------------------------------------------------- launch-per-half-sweep form ----
 // The same tiles, one workgroup per tile and one launch per half sweep: the kernel boundary is
-// the barrier between colours. Sweep index = sweeps_done (device) so captured graphs replay.
+// the barrier between colours. The sweep index lives in device memory, so captured graphs
+// replay with it advancing: a colour-0 launch reads sweeps_done[0] and copies it to [1], the
+// colour-1 launch reads [1] and sets [0] to the next sweep.
 __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
 {
     __shared__ MscQueueLds<2> lds;
     __shared__ u32 thr_lds[8];
     const u32 tile = xcd_band_block(blockIdx.x, gridDim.x);
-    const u32 sw = (u32)*ka.sweeps_done;
+    const u32 sw = (u32)ka.sweeps_done[color];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (color == 0u) ka.sweeps_done[1] = sw;
+        else ka.sweeps_done[0] = sw + 1u;
+    }
     const u32 par = sw & 1u;
     const u32 lat = ka.tile_lat[tile];
     const BatchDesc& d = ka.descs[lat];
@@ -256,15 +265,13 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
     block_store_sum(ka.plus + ((size_t)par * 2 + color) * ka.total_tiles + tile, plus);
 }
 
-// After both colours of a sweep, one workgroup of 1024 threads: a wave per lattice accumulates,
-// then the sweep counter advances.
+// After both colours of a sweep (when moments are on), one workgroup of 1024 threads: a wave
+// per lattice accumulates the sweep that the colour-1 launch just finished.
 __global__ __launch_bounds__(1024) void batch_accumulate(BatchKernelArgs ka)
 {
-    const u32 sw = (u32)*ka.sweeps_done;
+    const u32 sw = (u32)ka.sweeps_done[1];
     for (u32 lat = threadIdx.x >> 6; lat < ka.n_lat; lat += blockDim.x >> 6)
         batch_accumulate_lattice(ka, lat, sw & 1u);
-    __syncthreads();  // every thread has read the counter
-    if (threadIdx.x == 0) *ka.sweeps_done = sw + 1u;
 }
 
 __global__ void batch_zero_u64(unsigned long long* p, u32 n)
@@ -286,7 +293,8 @@ struct BatchHandle {
     unsigned long long* d_done = nullptr;
     int grid = 0, max_blocks = 0, n_cu = 0, per_cu = 0;
     u32 msc_mode = 0, msc_cap = 4;
-    int persistent = 0;  // 1: one cooperative launch per call; 0: three launches per sweep
+    int persistent = 0;  // 1: one cooperative launch per call; 0: two launches per sweep
+    int moments = 1;     // launch form: 1 adds the accumulation launch after every sweep
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     long long graph_sweeps = 0;
@@ -417,7 +425,7 @@ static int batch_alloc(BatchHandle* h, u32 cap_lat, u32 cap_tiles, int blocks_pe
     ISING_HIP(hipMalloc((void**)&h->d_tile_lat, sizeof(u32) * (size_t)cap_tiles));
     ISING_HIP(hipMalloc((void**)&h->d_plus, sizeof(u32) * 4 * (size_t)cap_tiles));
     ISING_HIP(hipMalloc((void**)&h->d_acc, sizeof(BatchAcc) * (size_t)cap_lat));
-    ISING_HIP(hipMalloc((void**)&h->d_done, sizeof(unsigned long long)));
+    ISING_HIP(hipMalloc((void**)&h->d_done, 2 * sizeof(unsigned long long)));
     return ISING_OK;
 }
 
@@ -436,7 +444,7 @@ static int batch_zero_stats(BatchHandle* h, hipStream_t st)
     if (h->n_lat)
         hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(256), 0, st, (unsigned long long*)h->d_acc,
                            (u32)(sizeof(BatchAcc) / 8) * h->n_lat);
-    hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(64), 0, st, h->d_done, 1u);
+    hipLaunchKernelGGL(batch_zero_u64, dim3(1), dim3(64), 0, st, h->d_done, 2u);
     ISING_LAUNCHED("batch_zero_u64");
     return ISING_OK;
 }
@@ -464,7 +472,7 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
     for (u32 s = 0; s < n_sweeps; ++s) {
         hipLaunchKernelGGL(batch_half, dim3(h->total_tiles), dim3(256), 0, st, ka, 0u);
         hipLaunchKernelGGL(batch_half, dim3(h->total_tiles), dim3(256), 0, st, ka, 1u);
-        hipLaunchKernelGGL(batch_accumulate, dim3(1), dim3(1024), 0, st, ka);
+        if (h->moments) hipLaunchKernelGGL(batch_accumulate, dim3(1), dim3(1024), 0, st, ka);
     }
     ISING_LAUNCHED("batch_half");
     return ISING_OK;
@@ -671,6 +679,18 @@ int ising_batch_set_persistent(void* handle, int persistent)
         return ising_fail(ISING_EINVAL, "batch_set_persistent: bad argument");
     batch_drop_graph(h);
     h->persistent = persistent;
+    return ISING_OK;
+}
+
+// 1 (default) = accumulate the moments of M after every sweep; 0 = skip that work (one launch
+// per sweep fewer in the launch form; the persistent form always accumulates). Lattices are
+// the same either way. Destroys a captured graph.
+int ising_batch_set_moments(void* handle, int on)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || (on != 0 && on != 1)) return ising_fail(ISING_EINVAL, "batch_set_moments: bad argument");
+    batch_drop_graph(h);
+    h->moments = on;
     return ISING_OK;
 }
 

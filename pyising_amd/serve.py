@@ -350,11 +350,12 @@ class Server:
     ``max_slots``: lattices per batched step; ``step_sweeps``: sweeps per step (shorter steps
     let arrivals join sooner); ``max_spins``: largest request accepted (sizes the upload
     staging buffer); ``msc_mode``: msc tile kind of the batched kernel (0 compaction, 1
-    lockstep).
+    lockstep: the default, since served msc lattices are small, at most ``max_spins``, and their
+    rows of few words leave compaction tiles short of full waves).
     """
 
     def __init__(self, budget_bytes: int = 4 << 30, max_slots: int = 256, step_sweeps: int = 25,
-                 max_spins: int = 1 << 24, msc_mode: int = 0, max_bypass: int = 16):
+                 max_spins: int = 1 << 24, msc_mode: int = 1, max_bypass: int = 16):
         from . import _lib
         from .batch import ReservedBatch
 

@@ -111,14 +111,14 @@ def test_batch_moments_are_exact(hip, persistent):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("persistent", [False, True])
-def test_batch_graph_replays_continue_the_stream(hip, persistent):
+@pytest.mark.parametrize("persistent,moments", [(False, True), (True, True), (False, False)])
+def test_batch_graph_replays_continue_the_stream(hip, persistent, moments):
     from pyising_amd.batch import BatchScan
 
     specs = _mixed(hip)[:6]
     sims = [_make(s) for s in specs]
     refs = [_make(s) for s in specs]
-    scan = BatchScan(sims, persistent=persistent)
+    scan = BatchScan(sims, persistent=persistent, moments=moments)
     if not scan.capture(2):
         scan.close()
         pytest.skip("the runtime cannot capture a cooperative launch into a graph")
@@ -128,7 +128,8 @@ def test_batch_graph_replays_continue_the_stream(hip, persistent):
     for sim, ref in zip(sims, refs, strict=True):
         ref.sweep(7)
         np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
-    assert scan.moments()[0].samples == 7
+    assert scan.moments()[0].samples == (7 if moments else 0)
+    assert scan.sweeps_done == 7
     scan.close()
 
 
@@ -155,6 +156,7 @@ def test_struct_layout_matches_header():
     assert IsingBatchLattice.table.offset == 32
     assert IsingBatchLattice.c0.offset == 64
     assert IsingBatchLattice.t0.offset == 80
+    assert IsingBatchLattice.vec.offset == 84
 
 
 def test_moments_math():
