@@ -44,6 +44,7 @@ struct BatchDesc {
     void* c1;
     u32 kind, ndim, hb, vec;  // vec: int8 sites per thread (16, 8 or 4)
     u32 msc_queue;            // msc: 1 = queue-compaction 512-word tiles, 0 = lockstep 256
+    u32 layout;               // msc: MSC_GENERAL, MSC_ROWW or MSC_SEGW classification
     Int8Geom g;
     Int8Table tb;
     u32 rows, nw, n_words;
@@ -110,15 +111,38 @@ __device__ __forceinline__ void batch_accumulate_lattice(const BatchKernelArgs& 
     }
 }
 
-// One 512-word msc tile by queue compaction (4 unrolled calls, then packed rounds); the
-// one-row-per-wave classification when the lattice's Lx % 8192 == 0 (block-uniform test).
-__device__ __forceinline__ u32 batch_msc_tile(const MscArgs& a, u64 codes, u32 base,
+// One 512-word msc tile by queue compaction (4 unrolled calls, then packed rounds), with the
+// lattice's wave classification (block-uniform branch).
+__device__ __forceinline__ u32 batch_msc_tile(const MscArgs& a, u64 codes, u32 base, u32 layout,
                                               MscQueueLds<2>& lds)
 {
     u32 calls = 0, slots = 0, words = 0;
-    if (a.nw % 64u == 0u)
-        return msc_queue_tile<2, 4, true, false, true, true>(a, codes, base, lds, calls, slots, words);
-    return msc_queue_tile<2, 4, false, false, true, true>(a, codes, base, lds, calls, slots, words);
+    if (layout == MSC_ROWW)
+        return msc_queue_tile<2, 4, MSC_ROWW, false, true, true>(a, codes, base, lds, calls, slots, words);
+    if (layout == MSC_SEGW)
+        return msc_queue_tile<2, 4, MSC_SEGW, false, true, true>(a, codes, base, lds, calls, slots, words);
+    return msc_queue_tile<2, 4, MSC_GENERAL, false, true, true>(a, codes, base, lds, calls, slots, words);
+}
+
+// One 256-word lockstep msc tile; returns the +1 spins written. With a whole-wave layout the
+// word count is a multiple of 64, so the validity test is wave-uniform.
+__device__ __forceinline__ u32 batch_msc_lockstep(const MscArgs& a, u64 codes, u32 local,
+                                                  u32 layout)
+{
+    const u32 w = local * 256u + threadIdx.x;
+    MscState st = {};
+    if (w < a.n_words) {
+        if (layout == MSC_ROWW) st = msc_classify_wave<MSC_ROWW>(a, w);
+        else if (layout == MSC_SEGW) st = msc_classify_wave<MSC_SEGW>(a, w);
+        else st = msc_classify(a, w);
+    }
+    u32 calls = 0, slots = 0;
+    msc_run<true, false>(a, codes, w, 0u, 16u, st, calls, slots);
+    if (w < a.n_words) {
+        a.own[w] = (u64)st.pwl | ((u64)st.pwh << 32);
+        return __popc(st.pwl) + __popc(st.pwh);
+    }
+    return 0u;
 }
 
 // One int8 tile; thr_lds is this workgroup's threshold table for the lattice (filled by
@@ -183,19 +207,8 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
                     a.color = color;
                     a.k0 = d.k0;
                     a.k1 = d.k1;
-                    u32 calls = 0, slots = 0;
-                    if (d.msc_queue) {
-                        plus = batch_msc_tile(a, d.codes, local * 512u, lds);
-                    } else {
-                        const u32 w = local * 256u + threadIdx.x;
-                        MscState st = {};
-                        if (w < a.n_words) st = msc_classify(a, w);
-                        msc_run<true, false>(a, d.codes, w, 0u, 16u, st, calls, slots);
-                        if (w < a.n_words) {
-                            a.own[w] = (u64)st.pwl | ((u64)st.pwh << 32);
-                            plus = __popc(st.pwl) + __popc(st.pwh);
-                        }
-                    }
+                    plus = d.msc_queue ? batch_msc_tile(a, d.codes, local * 512u, d.layout, lds)
+                                       : batch_msc_lockstep(a, d.codes, local, d.layout);
                 }
                 block_store_sum(ka.plus + ((size_t)par * 2 + color) * ka.total_tiles + tile, plus);
             }
@@ -248,19 +261,8 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
         a.color = color;
         a.k0 = d.k0;
         a.k1 = d.k1;
-        u32 calls = 0, slots = 0;
-        if (d.msc_queue) {
-            plus = batch_msc_tile(a, d.codes, local * 512u, lds);
-        } else {
-            const u32 w = local * 256u + threadIdx.x;
-            MscState st = {};
-            if (w < a.n_words) st = msc_classify(a, w);
-            msc_run<true, false>(a, d.codes, w, 0u, 16u, st, calls, slots);
-            if (w < a.n_words) {
-                a.own[w] = (u64)st.pwl | ((u64)st.pwh << 32);
-                plus = __popc(st.pwl) + __popc(st.pwh);
-            }
-        }
+        plus = d.msc_queue ? batch_msc_tile(a, d.codes, local * 512u, d.layout, lds)
+                           : batch_msc_lockstep(a, d.codes, local, d.layout);
     }
     block_store_sum(ka.plus + ((size_t)par * 2 + color) * ka.total_tiles + tile, plus);
 }
@@ -396,6 +398,7 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
             d.div_nw = make_fastdiv(d.nw);
             d.codes = msc_level_codes(L.table[3], L.table[4]);
             d.msc_queue = L.kind == BATCH_MSC_QUEUE || (L.kind == BATCH_MSC && msc_mode == 0);
+            d.layout = (u32)msc_layout(d.nw, d.n_words);
             const u32 per = d.msc_queue ? 512u : 256u;
             d.n_tiles = (d.n_words + per - 1u) / per;
         } else {

@@ -119,6 +119,58 @@ __device__ __forceinline__ MscState msc_classify_row(const MscRowLoad& L)
     return msc_state(L.o, L.c, (u64)hzl | ((u64)hzh << 32), L.up, L.dn);
 }
 
+// Classification when nw divides 64 and the lattice has a multiple of 64 words: the calling
+// wave's 64 words w0 .. w0 + 63 (w0 % 64 == 0, wave-uniform) are 64 / nw whole rows, lane l
+// holding word x = l % nw of row y0 + l / nw. Rows alternate parity inside the wave, so the
+// side of the horizontal neighbour is per lane; the boundary half-word comes from the lane
+// holding the word beside this one in the same row (wrapping within the row: the torus), by
+// one ds_bpermute. Every lane of the wave must execute it.
+struct MscSegLoad {
+    u64 o, c, up, dn;
+    u32 right;  // per lane: 1 = colour sites at odd columns, neighbour j + 1
+};
+
+__device__ __forceinline__ MscSegLoad msc_load_seg(const MscArgs& a, u32 w0, u32 lane)
+{
+    w0 = uni(w0);
+    const u32 lg = (u32)__builtin_ctz(a.nw);  // nw = 2^lg <= 32
+    const u32 y = (w0 >> lg) + (lane >> lg);
+    const u32 x = lane & (a.nw - 1u);
+    const u32 yu = (y + 1u == a.rows) ? 0u : y + 1u;
+    const u32 yd = (y == 0u) ? a.rows - 1u : y - 1u;
+    MscSegLoad L;
+    L.o = (a.own + w0)[lane];
+    L.c = (a.other + w0)[lane];
+    L.up = a.other[(yu << lg) + x];
+    L.dn = a.other[(yd << lg) + x];
+    L.right = (y + a.color) & 1u;
+    return L;
+}
+
+__device__ __forceinline__ MscState msc_classify_seg(const MscSegLoad& L, u32 nw, u32 lane)
+{
+    const u32 cl = (u32)L.c, ch = (u32)(L.c >> 32);
+    const u32 x = lane & (nw - 1u);
+    const u32 src = (lane - x) + ((x + (L.right ? 1u : nw - 1u)) & (nw - 1u));
+    // right: bit 0 of the next word (its low half); left: bit 63 of the previous word (its
+    // high half). The source lane is in the same row, so it has the same side.
+    const u32 nb = (u32)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)(L.right ? cl : ch));
+    const u32 hzl = L.right ? __builtin_amdgcn_alignbit(ch, cl, 1u) : __builtin_amdgcn_alignbit(cl, nb, 31u);
+    const u32 hzh = L.right ? __builtin_amdgcn_alignbit(nb, ch, 1u) : __builtin_amdgcn_alignbit(ch, cl, 31u);
+    return msc_state(L.o, L.c, (u64)hzl | ((u64)hzh << 32), L.up, L.dn);
+}
+
+// Which classification a lattice's waves use: whole-row waves (nw % 64 == 0), row-segment
+// waves (nw divides 64 and the word count is a multiple of 64), or per-lane general loads.
+enum : int { MSC_GENERAL = 0, MSC_ROWW = 1, MSC_SEGW = 2 };
+
+static inline int msc_layout(u32 nw, u32 n_words)
+{
+    if (nw % 64u == 0u) return MSC_ROWW;
+    if (nw && 64u % nw == 0u && n_words % 64u == 0u) return MSC_SEGW;
+    return MSC_GENERAL;
+}
+
 // Neighbour loads of word w (any geometry): five loads through the cache hierarchy, the
 // horizontal neighbour already shifted into place.
 struct MscLoad {
@@ -152,6 +204,18 @@ __device__ __forceinline__ MscState msc_classify(const MscLoad& L)
 __device__ __forceinline__ MscState msc_classify(const MscArgs& a, u32 w)
 {
     return msc_classify(msc_load(a, w));
+}
+
+// Classification of word w in a kernel whose waves hold 64 consecutive words starting at a
+// multiple of 64 (all lanes of the wave must call it when LAY != MSC_GENERAL; with n_words a
+// multiple of 64 a wave is wholly valid or wholly past the end, so callers branch per wave).
+template <int LAY>
+__device__ __forceinline__ MscState msc_classify_wave(const MscArgs& a, u32 w)
+{
+    const u32 lane = threadIdx.x & 63u;
+    if constexpr (LAY == MSC_ROWW) return msc_classify_row(msc_load_row(a, w - lane, lane));
+    else if constexpr (LAY == MSC_SEGW) return msc_classify_seg(msc_load_seg(a, w - lane, lane), a.nw, lane);
+    else return msc_classify(a, w);
 }
 
 // Philox call of word w at level pair lp (lp wave-uniform): counter (w, t, colour, 1 + lp).
@@ -365,24 +429,27 @@ __device__ __forceinline__ void msc_enqueue(MscQueueLds<R>& L, uint16_t* q, u32*
 // Returns, when COUNT, this thread's share of the number of +1 spins written.
 // Phase A's loads of one tile, issued before any of its arithmetic (the words of every round
 // are independent), so that memory latency hides behind earlier work.
-template <int R, bool ROWW>
+template <int R, int LAY>
 struct MscTileLoads {
-    MscRowLoad lr[ROWW ? R : 1];
-    MscLoad lg[ROWW ? 1 : R];
+    MscRowLoad lr[LAY == MSC_ROWW ? R : 1];
+    MscSegLoad ls[LAY == MSC_SEGW ? R : 1];
+    MscLoad lg[LAY == MSC_GENERAL ? R : 1];
 };
 
 // Branch-free: words past the end load a clamped (valid) address instead, and their results
 // are discarded by the caller.
-template <int R, bool ROWW>
-__device__ __forceinline__ void msc_tile_load(const MscArgs& a, u32 base, MscTileLoads<R, ROWW>& T)
+template <int R, int LAY>
+__device__ __forceinline__ void msc_tile_load(const MscArgs& a, u32 base, MscTileLoads<R, LAY>& T)
 {
     const u32 tid = threadIdx.x;
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
-        if (ROWW) {
+        if (LAY != MSC_GENERAL) {
             // n_words is a multiple of 64: clamp the wave's first word
             const u32 w0 = base + (u32)rr * 256u + (tid & ~63u);
-            T.lr[rr] = msc_load_row(a, w0 < a.n_words ? w0 : a.n_words - 64u, tid & 63u);
+            const u32 wc = w0 < a.n_words ? w0 : a.n_words - 64u;
+            if (LAY == MSC_ROWW) T.lr[rr] = msc_load_row(a, wc, tid & 63u);
+            else T.ls[rr] = msc_load_seg(a, wc, tid & 63u);
         } else {
             const u32 w = base + (u32)rr * 256u + tid;
             T.lg[rr] = msc_load(a, w < a.n_words ? w : a.n_words - 1u);
@@ -394,10 +461,10 @@ struct MscNoPrefetch {
     __device__ void operator()() const {}
 };
 
-template <int R, int CAP, bool ROWW, bool STATS, bool COUNT, bool NT, class PF>
+template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, class PF>
 __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes, u32 base,
                                                      MscQueueLds<R>& L,
-                                                     const MscTileLoads<R, ROWW>& T, PF prefetch,
+                                                     const MscTileLoads<R, LAY>& T, PF prefetch,
                                                      u32& calls, u32& slots, u32& words)
 {
     static_assert(CAP >= 1 && CAP <= 15, "phase A levels");
@@ -424,7 +491,10 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
             asm volatile("" : "+v"(r[lp].x), "+v"(r[lp].y), "+v"(r[lp].z), "+v"(r[lp].w));
         }
         __builtin_amdgcn_sched_barrier(0);
-        MscState s = ROWW ? msc_classify_row(T.lr[ROWW ? rr : 0]) : msc_classify(T.lg[ROWW ? 0 : rr]);
+        MscState s;
+        if constexpr (LAY == MSC_ROWW) s = msc_classify_row(T.lr[rr]);
+        else if constexpr (LAY == MSC_SEGW) s = msc_classify_seg(T.ls[rr], a.nw, lane);
+        else s = msc_classify(T.lg[rr]);
         if (!valid) {
             s.ul = 0u;  // nothing to decide, store or park
             s.uh = 0u;
@@ -496,13 +566,13 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
     return ones;
 }
 
-template <int R, int CAP, bool ROWW, bool STATS, bool COUNT, bool NT>
+template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT>
 __device__ __forceinline__ u32 msc_queue_tile(const MscArgs& a, u64 codes, u32 base,
                                               MscQueueLds<R>& L, u32& calls, u32& slots,
                                               u32& words)
 {
-    MscTileLoads<R, ROWW> T;
-    msc_tile_load<R, ROWW>(a, base, T);
-    return msc_queue_tile_loaded<R, CAP, ROWW, STATS, COUNT, NT>(a, codes, base, L, T,
+    MscTileLoads<R, LAY> T;
+    msc_tile_load<R, LAY>(a, base, T);
+    return msc_queue_tile_loaded<R, CAP, LAY, STATS, COUNT, NT>(a, codes, base, L, T,
                                                              MscNoPrefetch{}, calls, slots, words);
 }

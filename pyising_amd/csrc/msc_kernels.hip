@@ -42,14 +42,14 @@
 
 // Lockstep: one word per lane, the wave pays the maximum call count of its 64 words.
 // LAZY = false is the eager ablation (all 16 calls, src/pyising/backends/cuda/ising.cu:135).
-template <bool LAZY, bool STATS>
+template <bool LAZY, bool STATS, int LAY>
 __global__ __launch_bounds__(256) void msc_half_lockstep(MscArgs a, u64 codes,
                                                         unsigned long long* stats)
 {
     const u32 w = xcd_band_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const bool valid = w < a.n_words;
+    const bool valid = w < a.n_words;  // wave-uniform unless LAY == MSC_GENERAL
     MscState s = {};
-    if (valid) s = msc_classify(a, w);
+    if (valid) s = msc_classify_wave<LAY>(a, w);
     u32 calls = 0, slots = 0;
     msc_run<LAZY, STATS>(a, codes, w, 0u, 16u, s, calls, slots);
     if (valid) a.own[w] = (u64)s.pwl | ((u64)s.pwh << 32);
@@ -64,7 +64,7 @@ static_assert(sizeof(MscQueueLds<1>) <= sizeof(MscQueueLds<2>), "graded tiles sh
 // Graded tail: workgroups [0, n_big) take 256 R-word tiles (XCD band mapped); the ones after,
 // dispatched last, take 256-word tiles (R = 2 only), so the final generation of workgroups
 // finishes sooner and the launch's tail is shorter.
-template <int R, int CAP, bool ROWW, bool STATS>
+template <int R, int CAP, int LAY, bool STATS>
 __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u64 codes, u32 n_big,
                                                      unsigned long long* stats)
 {
@@ -72,10 +72,10 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u64 codes, u32 
     u32 calls = 0, slots = 0, words = 0;
     if (R == 1 || blockIdx.x < n_big) {
         const u32 base = xcd_band_block(blockIdx.x, n_big) * (256u * R);
-        msc_queue_tile<R, CAP, ROWW, STATS, false, true>(a, codes, base, lds, calls, slots, words);
+        msc_queue_tile<R, CAP, LAY, STATS, false, true>(a, codes, base, lds, calls, slots, words);
     } else if constexpr (R == 2) {
         const u32 base = n_big * 512u + (blockIdx.x - n_big) * 256u;
-        msc_queue_tile<1, CAP, ROWW, STATS, false, true>(
+        msc_queue_tile<1, CAP, LAY, STATS, false, true>(
             a, codes, base, *reinterpret_cast<MscQueueLds<1>*>(&lds), calls, slots, words);
     }
     if (STATS) msc_stats(stats, calls, slots, words);
@@ -179,7 +179,7 @@ static int msc_mode_ok(int mode, int k)
     return ISING_OK;
 }
 
-template <int R, int CAP, bool ROWW>
+template <int R, int CAP, int LAY>
 static void launch_queue3(u32 n_tiles, hipStream_t st, const MscArgs& a, u64 codes,
                           unsigned long long* stats)
 {
@@ -193,17 +193,32 @@ static void launch_queue3(u32 n_tiles, hipStream_t st, const MscArgs& a, u64 cod
         grid = n_big + (a.n_words - n_big * 512u + 255u) / 256u;
     }
     if (stats)
-        hipLaunchKernelGGL((msc_half_queue<R, CAP, ROWW, true>), dim3(grid), dim3(256), 0, st, a, codes, n_big, stats);
+        hipLaunchKernelGGL((msc_half_queue<R, CAP, LAY, true>), dim3(grid), dim3(256), 0, st, a, codes, n_big, stats);
     else
-        hipLaunchKernelGGL((msc_half_queue<R, CAP, ROWW, false>), dim3(grid), dim3(256), 0, st, a, codes, n_big, stats);
+        hipLaunchKernelGGL((msc_half_queue<R, CAP, LAY, false>), dim3(grid), dim3(256), 0, st, a, codes, n_big, stats);
 }
 
 template <int R, int CAP>
 static void launch_queue2(u32 n_tiles, hipStream_t st, const MscArgs& a, u64 codes,
                           unsigned long long* stats)
 {
-    if (a.nw % 64u == 0u) launch_queue3<R, CAP, true>(n_tiles, st, a, codes, stats);
-    else launch_queue3<R, CAP, false>(n_tiles, st, a, codes, stats);
+    const int lay = msc_layout(a.nw, a.n_words);
+    if (lay == MSC_ROWW) launch_queue3<R, CAP, MSC_ROWW>(n_tiles, st, a, codes, stats);
+    else if (lay == MSC_SEGW) launch_queue3<R, CAP, MSC_SEGW>(n_tiles, st, a, codes, stats);
+    else launch_queue3<R, CAP, MSC_GENERAL>(n_tiles, st, a, codes, stats);
+}
+
+template <int LAY>
+static void launch_lockstep(dim3 grid, hipStream_t st, const MscArgs& a, u64 codes, bool lazy,
+                            unsigned long long* stats)
+{
+    if (lazy) {
+        if (stats) hipLaunchKernelGGL((msc_half_lockstep<true, true, LAY>), grid, dim3(256), 0, st, a, codes, stats);
+        else hipLaunchKernelGGL((msc_half_lockstep<true, false, LAY>), grid, dim3(256), 0, st, a, codes, stats);
+    } else {
+        if (stats) hipLaunchKernelGGL((msc_half_lockstep<false, true, LAY>), grid, dim3(256), 0, st, a, codes, stats);
+        else hipLaunchKernelGGL((msc_half_lockstep<false, false, LAY>), grid, dim3(256), 0, st, a, codes, stats);
+    }
 }
 
 template <int R>
@@ -244,13 +259,10 @@ static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32
         else launch_queue<4>(n_tiles, st, a, codes, cap, stats);
     } else {
         const dim3 grid((a.n_words + 255u) / 256u);
-        if (mode == 1) {
-            if (stats) hipLaunchKernelGGL((msc_half_lockstep<true, true>), grid, dim3(256), 0, st, a, codes, stats);
-            else hipLaunchKernelGGL((msc_half_lockstep<true, false>), grid, dim3(256), 0, st, a, codes, stats);
-        } else {
-            if (stats) hipLaunchKernelGGL((msc_half_lockstep<false, true>), grid, dim3(256), 0, st, a, codes, stats);
-            else hipLaunchKernelGGL((msc_half_lockstep<false, false>), grid, dim3(256), 0, st, a, codes, stats);
-        }
+        const int lay = msc_layout(a.nw, a.n_words);
+        if (lay == MSC_ROWW) launch_lockstep<MSC_ROWW>(grid, st, a, codes, mode == 1, stats);
+        else if (lay == MSC_SEGW) launch_lockstep<MSC_SEGW>(grid, st, a, codes, mode == 1, stats);
+        else launch_lockstep<MSC_GENERAL>(grid, st, a, codes, mode == 1, stats);
     }
     ISING_LAUNCHED("msc_half");
     return ISING_OK;

@@ -41,8 +41,13 @@ MSC_MODES = [(0, 42), (0, 32), (0, 52), (0, 62), (0, 44), (0, 34), (1, 0), (2, 0
 
 
 @pytest.mark.parametrize("mode,k", MSC_MODES)
+# classification layouts: rows of nw = Lx/128 words; nw % 64 == 0 -> whole-row waves
+# ((16, 8192), (8, 16384)); nw dividing 64 with a multiple of 64 words -> row-segment waves
+# ((128, 128), (16, 1024), (256, 256), (64, 512), (64, 2048), (32, 4096)); otherwise per-lane
+# loads ((64, 384), (130, 256), (66, 128), (6, 24576))
 @pytest.mark.parametrize("shape", [(128, 128), (64, 384), (16, 1024), (130, 256),
-                                   (16, 8192), (8, 16384), (6, 24576)])
+                                   (16, 8192), (8, 16384), (6, 24576), (256, 256), (64, 512),
+                                   (64, 2048), (32, 4096), (66, 128)])
 def test_msc_bitwise_identical(hip, shape, mode, k):
     lat = O.random_lattice(shape, 9)
     ref = O.MscOracle(shape, 0.44, seed=11, init=lat)
