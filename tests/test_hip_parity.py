@@ -231,6 +231,34 @@ def test_config_int8_L2048_temperatures(hip, T):
     np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
 
 
+@pytest.mark.parametrize("T", [1.5, 2.269, 3.0])
+def test_config_int8_L8192_temperatures(hip, T):
+    """2D int8 Metropolis at the benchmark size, L=8192, T in {1.5, 2.269, 3.0}, seed=1, hot."""
+    shape = (8192, 8192)
+    sim = hip.HipInt8(shape, 1 / T, seed=1)
+    ref = CO.CInt8(shape, 1 / T, seed=1)
+    sim.sweep(4)
+    ref.sweep(4)
+    assert sim.observables_exact() == ref.observables_exact()
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+
+
+@pytest.mark.parametrize("algorithm", ["metropolis", "heatbath"])
+@pytest.mark.parametrize("K,init", [(0.221654626, "hot"), (0.221654626, "cold"), (0.20, "cold"),
+                                    (0.24, "hot")])
+def test_config_3d_int8_L256(hip, algorithm, K, init):
+    """3D int8 at the configuration's own size, L=256 (16.8 M spins), seed=7: (M, E) every 10
+    sweeps and the lattice after 30."""
+    shape = (256, 256, 256)
+    sim = hip.HipInt8(shape, K, seed=7, algorithm=algorithm, init=init)
+    ref = CO.CInt8(shape, K, seed=7, algorithm=algorithm, init=init)
+    for _ in range(3):
+        sim.sweep(10)
+        ref.sweep(10)
+        assert sim.observables_exact() == ref.observables_exact()
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+
+
 # sites per lane follow the lattice size (the widest of 16/8/4 dividing Lx/2 that gives >= 2^17
 # lanes): (2048, 1024) and (64, 128, 256) take 8, (128, 128, 256) takes 16, (32, 32, 32) and
 # (1024, 1024) take 4, (2048, 2048) takes 16; every width must give the same spins
