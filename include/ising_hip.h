@@ -40,6 +40,10 @@
 extern "C" {
 #endif
 
+/* Scratch a caller passes to the observable reductions (ising_int8_obs, ising_msc_obs): one
+ * pair of integer partial sums per workgroup, any contents on entry. */
+#define ISING_OBS_SCRATCH_BYTES 16384
+
 /* ---------------------------------------------------------------- status, runtime ---- */
 /* Message of the calling thread's last failure (empty if none). No reference counterpart. */
 const char* ising_last_error(void);
@@ -109,7 +113,8 @@ int ising_int8_half(int8_t* d_own, const int8_t* d_other, long long rows, int lh
 int ising_int8_sweeps(int8_t* d_c0, int8_t* d_c1, long long rows, int lh, int ly, int lz, int ndim,
                       int heatbath, const uint32_t* table, uint32_t t0, long long n_sweeps,
                       uint32_t k0, uint32_t k1, void* stream);
-/* Exact (M, E) into out2[0..1] (device int64); scratch2 = 2 device uint64 (any contents).
+/* Exact (M, E) into out2[0..1] (device int64); d_scratch2 = ISING_OBS_SCRATCH_BYTES of device
+ * memory (any contents).
  * observables_int8, src/pyising/backends/observables_torch.py:46-69. */
 int ising_int8_obs(const int8_t* d_c0, const int8_t* d_c1, long long rows, int lh, int ly, int lz,
                    int ndim, unsigned long long* d_scratch2, long long* d_out2, void* stream);
@@ -145,7 +150,8 @@ int ising_msc_sweeps_ex(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw, 
                         long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4,
                         uint32_t thr8, int mode, int k, unsigned long long* d_stats,
                         void* stream);
-/* Exact (M, E) of a packed lattice: observables_packed, observables_torch.py:85-111. */
+/* Exact (M, E) of a packed lattice: observables_packed, observables_torch.py:85-111.
+ * d_scratch2 = ISING_OBS_SCRATCH_BYTES of device memory (any contents). */
 int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, int nw,
                   unsigned long long* d_scratch2, long long* d_out2, void* stream);
 

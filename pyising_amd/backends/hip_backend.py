@@ -26,6 +26,8 @@ _ALL_ONES = 0xFFFFFFFFFFFFFFFF
 
 #: msc kernel modes of ising_msc_sweeps_ex
 MSC_DEFAULT, MSC_COMPACT, MSC_LOCKSTEP, MSC_EAGER, MSC_BY_SIZE = -1, 0, 1, 2, 3
+#: ISING_OBS_SCRATCH_BYTES of ising_hip.h: partial sums of the observable reductions
+OBS_SCRATCH_BYTES = 16384
 
 
 def _full_lattice(lat: np.ndarray, shape) -> np.ndarray:
@@ -86,16 +88,16 @@ class _HipBase(IsingSimulation):
     def _hip_setup(self) -> None:
         self._L = _lib.lib()
         self.k0, self.k1 = split_seed(self.seed)
-        self._obs_scratch = self._alloc(16)
+        self._obs_scratch = self._alloc(OBS_SCRATCH_BYTES)
         self._obs_out = self._alloc(16)
 
     @staticmethod
     def state_bytes(shape, variant: str) -> list[int]:
         """Sizes of the device allocations a simulation of ``shape`` makes through its
-        allocator: two colour arrays and two 16-byte reduction buffers."""
+        allocator: the reduction scratch, the 16-byte (M, E) result and two colour arrays."""
         n = int(np.prod(shape))
         colour = n // 16 if variant == "msc" else n // 2
-        return [16, 16, colour, colour]
+        return [OBS_SCRATCH_BYTES, 16, colour, colour]
 
     def synchronize(self) -> None:
         _lib.check(self._L.ising_stream_sync(self.stream_ptr), "stream_sync")

@@ -267,13 +267,13 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
     block_store_sum(ka.plus + ((size_t)par * 2 + color) * ka.total_tiles + tile, plus);
 }
 
-// After both colours of a sweep (when moments are on), one workgroup of 1024 threads: a wave
-// per lattice accumulates the sweep that the colour-1 launch just finished.
-__global__ __launch_bounds__(1024) void batch_accumulate(BatchKernelArgs ka)
+// After both colours of a sweep (when moments are on): one wave per lattice, four per
+// workgroup, spread over the CUs, accumulates the sweep that the colour-1 launch just finished.
+__global__ __launch_bounds__(256) void batch_accumulate(BatchKernelArgs ka)
 {
     const u32 sw = (u32)ka.sweeps_done[1];
-    for (u32 lat = threadIdx.x >> 6; lat < ka.n_lat; lat += blockDim.x >> 6)
-        batch_accumulate_lattice(ka, lat, sw & 1u);
+    const u32 lat = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (lat < ka.n_lat) batch_accumulate_lattice(ka, lat, sw & 1u);
 }
 
 __global__ void batch_zero_u64(unsigned long long* p, u32 n)
@@ -475,7 +475,8 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
     for (u32 s = 0; s < n_sweeps; ++s) {
         hipLaunchKernelGGL(batch_half, dim3(h->total_tiles), dim3(256), 0, st, ka, 0u);
         hipLaunchKernelGGL(batch_half, dim3(h->total_tiles), dim3(256), 0, st, ka, 1u);
-        if (h->moments) hipLaunchKernelGGL(batch_accumulate, dim3(1), dim3(1024), 0, st, ka);
+        if (h->moments)
+            hipLaunchKernelGGL(batch_accumulate, dim3((h->n_lat + 3u) / 4u), dim3(256), 0, st, ka);
     }
     ISING_LAUNCHED("batch_half");
     return ISING_OK;

@@ -197,4 +197,57 @@ __device__ __forceinline__ u32 xcd_band_block(u32 b, u32 nblocks)
     return (b & 7u) * per + (b >> 3);
 }
 
+// ------------------------------------------------------------- exact reductions ----
+// Observables reduce without atomics: each workgroup (256 threads) of a grid of at most
+// OBS_MAX_BLOCKS writes its two integer partial sums to its own slots of the caller's scratch
+// (ISING_OBS_SCRATCH_BYTES), and one finalize workgroup adds them up. Device-scope atomics on
+// one address serialize at the memory side (one per wave cost ~0.2 ms on a 256^3 lattice).
+constexpr u32 OBS_MAX_BLOCKS = ISING_OBS_SCRATCH_BYTES / 16u;
+
+// Sum of a and b over the 256 threads of the workgroup (wave shuffles, then LDS); every thread
+// must call it; thread 0 receives the totals.
+__device__ __forceinline__ void block_sum2_u64(u64& a, u64& b)
+{
+    __shared__ u64 part[2][4];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        part[0][threadIdx.x >> 6] = a;
+        part[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+        b = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+    }
+}
+
+// out[0] = ka * A + ca, out[1] = kb * B + cb with (A, B) the sums of the nblocks partial pairs
+// (one 256-thread workgroup). Nothing needs clearing between calls: every partial is rewritten.
+static __global__ void __launch_bounds__(256) obs_finalize(const unsigned long long* partials,
+                                                          u32 nblocks, long long* out,
+                                                          long long ka, long long ca, long long kb,
+                                                          long long cb)
+{
+    u64 a = 0, b = 0;
+    for (u32 i = threadIdx.x; i < nblocks; i += 256u) {
+        a += partials[2u * i];
+        b += partials[2u * i + 1u];
+    }
+    block_sum2_u64(a, b);
+    if (threadIdx.x == 0) {
+        out[0] = ka * (long long)a + ca;
+        out[1] = kb * (long long)b + cb;
+    }
+}
+
+static inline unsigned obs_grid(u64 work_items)
+{
+    const u64 blocks = (work_items + 255u) / 256u;
+    return (unsigned)(blocks < OBS_MAX_BLOCKS ? (blocks ? blocks : 1u) : OBS_MAX_BLOCKS);
+}
+
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }

@@ -46,19 +46,13 @@ __global__ __launch_bounds__(256) void int8_half_kernel(int8_t* __restrict__ own
 // M = sum of all spins = N - 2 * (#-1); E = -sum over colour-0 sites of s * h =
 // 2 * (#anti-aligned bonds) - d * N (every bond has exactly one colour-0 end,
 // src/pyising/backends/observables_torch.py:1-4). Accumulated as exact integers: per-lane
-// counts, wave reduction with cross-lane adds, one 64-bit atomic per wave.
-__device__ __forceinline__ u64 wave_sum_u64(u64 v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
+// counts, wave reduction with cross-lane adds, LDS across the workgroup's waves, one partial
+// pair per workgroup, summed by obs_finalize.
 
 template <int NDIM, int V>
 __global__ __launch_bounds__(256) void int8_obs_kernel(const int8_t* __restrict__ c0,
                                                        const int8_t* __restrict__ c1, Int8Geom g,
-                                                       unsigned long long* __restrict__ acc)
+                                                       unsigned long long* __restrict__ partials)
 {
     constexpr int NWd = V / 4;
     u64 minus = 0, anti = 0;
@@ -108,30 +102,11 @@ __global__ __launch_bounds__(256) void int8_obs_kernel(const int8_t* __restrict_
         minus += cm;
         anti += ca;
     }
-    minus = wave_sum_u64(minus);
-    anti = wave_sum_u64(anti);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(acc + 0, (unsigned long long)minus);
-        atomicAdd(acc + 1, (unsigned long long)anti);
+    block_sum2_u64(minus, anti);
+    if (threadIdx.x == 0) {
+        partials[2u * blockIdx.x] = minus;
+        partials[2u * blockIdx.x + 1u] = anti;
     }
-}
-
-// out[0] = M = N - 2 minus; out[1] = E = 2 anti - d N. Also clears the accumulator so the
-// next call starts from zero without a memset.
-__global__ void obs_finalize_kernel(unsigned long long* acc, long long* out, long long n_spins,
-                                    int ndim)
-{
-    const long long minus = (long long)acc[0], anti = (long long)acc[1];
-    out[0] = n_spins - 2 * minus;
-    out[1] = 2 * anti - (long long)ndim * n_spins;
-    acc[0] = 0;
-    acc[1] = 0;
-}
-
-__global__ void zero_u64_kernel(unsigned long long* p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------- host launchers ----
@@ -249,9 +224,7 @@ int ising_int8_obs(const int8_t* c0, const int8_t* c1, long long rows, int lh, i
     int rc = make_geom(rows, lh, ly, lz, ndim, &g, &V);
     if (rc) return rc;
     hipStream_t st = as_stream(stream);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, scratch2, 2);
-    const u64 blocks = (g.n_chunks + 255) / 256;
-    const dim3 grid((unsigned)(blocks < 2048 ? blocks : 2048));
+    const dim3 grid(obs_grid(g.n_chunks));
 #define OBS_LAUNCH(ND, VV) \
     hipLaunchKernelGGL((int8_obs_kernel<ND, VV>), grid, dim3(256), 0, st, c0, c1, g, scratch2)
     if (ndim == 2) {
@@ -264,8 +237,10 @@ int ising_int8_obs(const int8_t* c0, const int8_t* c1, long long rows, int lh, i
         else OBS_LAUNCH(3, 4);
     }
 #undef OBS_LAUNCH
+    // M = N - 2 minus, E = 2 anti - d N
     const long long n_spins = 2 * rows * (long long)lh;
-    hipLaunchKernelGGL(obs_finalize_kernel, dim3(1), dim3(1), 0, st, scratch2, out2, n_spins, ndim);
+    hipLaunchKernelGGL(obs_finalize, dim3(1), dim3(256), 0, st, scratch2, grid.x, out2, -2ll,
+                       n_spins, 2ll, -(long long)ndim * n_spins);
     ISING_LAUNCHED("int8_obs_kernel");
     return ISING_OK;
 }

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u64 codes, u32 
 __global__ __launch_bounds__(256) void msc_obs_kernel(const u64* __restrict__ w0,
                                                       const u64* __restrict__ w1, u32 rows, u32 nw,
                                                       u32 n_words, FastDiv div_nw,
-                                                      unsigned long long* acc)
+                                                      unsigned long long* partials)
 {
     u64 ones = 0, anti = 0;
     const u32 stride = gridDim.x * blockDim.x;
@@ -107,30 +107,11 @@ __global__ __launch_bounds__(256) void msc_obs_kernel(const u64* __restrict__ w0
         ones += __popcll(s) + __popcll(c);
         anti += __popcll(s ^ c) + __popcll(s ^ hz) + __popcll(s ^ up) + __popcll(s ^ dn);
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        ones += __shfl_xor(ones, off, 64);
-        anti += __shfl_xor(anti, off, 64);
+    block_sum2_u64(ones, anti);
+    if (threadIdx.x == 0) {
+        partials[2u * blockIdx.x] = ones;
+        partials[2u * blockIdx.x + 1u] = anti;
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(acc + 0, (unsigned long long)ones);
-        atomicAdd(acc + 1, (unsigned long long)anti);
-    }
-}
-
-// out = (M, E); also clears the accumulator for the next call.
-__global__ void msc_obs_finalize(unsigned long long* acc, long long* out, long long n_spins)
-{
-    const long long ones = (long long)acc[0], anti = (long long)acc[1];
-    out[0] = 2 * ones - n_spins;
-    out[1] = 2 * anti - 2 * n_spins;
-    acc[0] = 0;
-    acc[1] = 0;
-}
-
-__global__ void msc_zero2(unsigned long long* p)
-{
-    if (threadIdx.x < 2) p[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------- host launchers ----
@@ -348,12 +329,13 @@ int ising_msc_obs(const uint64_t* w0, const uint64_t* w1, long long rows, int nw
     if (!scratch2 || !out2) return ising_fail(ISING_EINVAL, "msc_obs: null pointer");
     hipStream_t st = as_stream(stream);
     const u64 n_words = (u64)rows * (u64)nw;
-    hipLaunchKernelGGL(msc_zero2, dim3(1), dim3(64), 0, st, scratch2);
-    const u64 blocks = (n_words + 255) / 256;
-    hipLaunchKernelGGL(msc_obs_kernel, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(256), 0,
-                       st, w0, w1, (u32)rows, (u32)nw, (u32)n_words, make_fastdiv((u32)nw), scratch2);
-    hipLaunchKernelGGL(msc_obs_finalize, dim3(1), dim3(1), 0, st, scratch2, out2,
-                       (long long)n_words * 128);
+    const unsigned grid = obs_grid(n_words);
+    hipLaunchKernelGGL(msc_obs_kernel, dim3(grid), dim3(256), 0, st, w0, w1, (u32)rows, (u32)nw,
+                       (u32)n_words, make_fastdiv((u32)nw), scratch2);
+    // M = 2 ones - N, E = 2 anti - 2 N
+    const long long n_spins = (long long)n_words * 128;
+    hipLaunchKernelGGL(obs_finalize, dim3(1), dim3(256), 0, st, scratch2, grid, out2, 2ll,
+                       -n_spins, 2ll, -2 * n_spins);
     ISING_LAUNCHED("msc_obs_kernel");
     return ISING_OK;
 }

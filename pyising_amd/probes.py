@@ -10,6 +10,8 @@ Command line (short, deterministic runs to put under ``rocprofv3``)::
     python -m pyising_amd.probes copy --mib 1024 --reps 4     # calibration for FETCH/WRITE_SIZE
     python -m pyising_amd.probes msc --L 16384 --sweeps 10     # msc half sweeps at T_c
     python -m pyising_amd.probes int8 --L 8192 --sweeps 10     # int8 2D half sweeps at T_c
+    python -m pyising_amd.probes batch --sweeps 50             # Binder-scan batch, 128 lattices
+    python -m pyising_amd.probes serve --requests 300          # serving loop, back to back
 """
 
 from __future__ import annotations
@@ -103,6 +105,32 @@ def _run_int8(L: int, sweeps: int, dim: int, algorithm: str) -> dict:
             "algorithmic_bytes_per_launch": int(3.0 * L**dim / 2)}
 
 
+def _run_batch(sweeps: int, msc_tile: str, moments: bool) -> dict:
+    """The Binder-scan batch (96 int8 lattices L = 64..2048, 32 msc lattices L = 1024, 4096,
+    16 temperatures each), one launch per half sweep, fixed tiles, no graph."""
+    from .batch import BatchScan, temperature_scan_lattices
+
+    sims = temperature_scan_lattices()
+    tiles = {(1024, 1024): msc_tile, (4096, 4096): msc_tile}
+    scan = BatchScan(sims, msc_tiles=tiles, moments=moments)
+    scan.sweep(sweeps)
+    scan.synchronize()
+    scan.close()
+    return {"probe": "batch", "lattices": len(sims), "spins": int(sum(s.n_spins for s in sims)),
+            "sweeps": sweeps, "msc_tile": msc_tile, "moments": moments}
+
+
+def _run_serve(requests: int, rate: float) -> dict:
+    """The serving loop on the Poisson request stream, served back to back (not real time)."""
+    from .serve import Server, poisson_requests
+
+    reqs = poisson_requests(requests, rate=rate, seed=0)
+    srv = Server(budget_bytes=8 << 30, max_slots=256, step_sweeps=25, max_spins=1 << 24)
+    rep = srv.run(reqs, realtime=False, timeout=600)
+    srv.close()
+    return {"probe": "serve", **rep.summary()}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -123,6 +151,13 @@ def main(argv=None) -> int:
     ph.add_argument("--calls", type=int, default=512)
     ph.add_argument("--reps", type=int, default=3)
     sub.add_parser("ceilings")
+    b = sub.add_parser("batch")
+    b.add_argument("--sweeps", type=int, default=50)
+    b.add_argument("--msc-tile", default="lockstep", choices=("lockstep", "queue"))
+    b.add_argument("--no-moments", action="store_true")
+    sv = sub.add_parser("serve")
+    sv.add_argument("--requests", type=int, default=300)
+    sv.add_argument("--rate", type=float, default=200.0)
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         out = _run_copy(a.mib, a.reps)
@@ -133,6 +168,10 @@ def main(argv=None) -> int:
         out = _run_msc(a.L, a.sweeps, a.mode, a.k)
     elif a.cmd == "int8":
         out = _run_int8(a.L, a.sweeps, a.dim, a.algorithm)
+    elif a.cmd == "batch":
+        out = _run_batch(a.sweeps, a.msc_tile, not a.no_moments)
+    elif a.cmd == "serve":
+        out = _run_serve(a.requests, a.rate)
     else:
         out = {"copy_bandwidth_GBs": copy_bandwidth(), "philox_calls_per_ns": philox_rate(),
                **device_info()}

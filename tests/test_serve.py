@@ -73,12 +73,14 @@ def test_arena_random_churn_keeps_invariants():
 
 # -------------------------------------------------------------------------- admission (CPU) --
 def test_footprint_is_the_aligned_state():
-    r = _req(0, L=64)  # int8 2D: 2 colour arrays of 64*32 bytes + 2 x 16 B
-    assert r.footprint() == 2 * 2048 + 2 * ALIGN
+    from pyising_amd.backends.hip_backend import OBS_SCRATCH_BYTES as OBS
+
+    r = _req(0, L=64)  # int8 2D: 2 colour arrays of 64*32 bytes, reduction scratch, 16 B result
+    assert r.footprint() == 2 * 2048 + OBS + ALIGN
     m = _req(1, L=256, variant="msc")  # 2 x 256*2 words of 8 B
-    assert m.footprint() == 2 * 4096 + 2 * ALIGN
+    assert m.footprint() == 2 * 4096 + OBS + ALIGN
     c = _req(2, dim=3, L=16)  # 2 x 16*16*8 bytes
-    assert c.footprint() == 2 * 2048 + 2 * ALIGN
+    assert c.footprint() == 2 * 2048 + OBS + ALIGN
 
 
 @pytest.mark.parametrize("bad", [
@@ -105,10 +107,11 @@ def test_oversize_and_duplicate_requests_are_rejected():
 
 
 def test_admission_by_memory_and_slots_fifo():
-    big, small = _req(0, L=128).footprint(), _req(0, L=32).footprint()
+    big, small = _req(0, L=256).footprint(), _req(0, L=32).footprint()
+    assert 3 * small < big
     arena = DeviceArena(2 * big)
     ctrl = AdmissionController(arena, max_slots=3, max_bypass=2)
-    for i, L in enumerate([128, 128, 128, 32, 32, 32, 32]):
+    for i, L in enumerate([256, 256, 256, 32, 32, 32, 32]):
         ctrl.submit(_req(i, L=L))
     got = [r.rid for r, _ in ctrl.admit()]
     assert got == [0, 1]  # memory full: 2 x big; head (2) blocked, no room for small either
