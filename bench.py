@@ -114,7 +114,7 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
 
 
 def serving(n_requests: int = 300, rate: float = 200.0, seed: int = 0) -> dict:
-    """The serving loop on the config-5 request stream: Poisson arrivals at ``rate`` per second
+    """The serving loop on the Poisson request stream: arrivals at ``rate`` per second
     (2D L = 32..4096 int8/msc with p = 0.7, else 3D L = 16..256 int8 Metropolis/heat bath;
     T in [0.8, 1.2] T_c; sweeps log-uniform in [100, 10^4]; hot starts), served in real time
     with continuous batching. Latency = finish - arrival; aggregate flips/ns over the wall time

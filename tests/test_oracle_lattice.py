@@ -3,7 +3,7 @@
 * layouts: round trips and the parity-grid selection (tests/test_lattice.py:19-30 of the
   reference);
 * thresholds: the reference's own values at beta = 0.5 (tests/test_lattice.py:33-38), plus
-  floor(p 2^32) recomputed with 60-digit decimal arithmetic at the configs' betas;
+  floor(p 2^32) recomputed with 60-digit decimal arithmetic at the workloads' betas;
 * int8 sweeps: ``naive_sweep`` on the full lattice (tests/test_lattice.py:41-72);
 * msc sweeps: the eager full-lattice statement (every 32-bit uniform assembled, int8 rule);
 * observables: brute-force bond sums and the cold-start ground state;
@@ -67,7 +67,7 @@ def test_thresholds_against_decimal_arithmetic(beta, ndim):
         em = (-2 * b * x).exp()
         assert int(t[i]) == (0xFFFFFFFF if x <= 0 else _exact_threshold(em)), (beta, x)
         assert int(h[i]) == _exact_threshold(1 / (1 + em)), (beta, x)
-    # the hot path's two msc thresholds at the configs' beta
+    # the hot path's two msc thresholds at beta = 0.4406868
     if beta == 0.4406868:
         assert (int(t[3]), int(t[4])) == (736899868, 126432026)
     if beta == 1 / P.TC_2D:
