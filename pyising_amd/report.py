@@ -320,6 +320,44 @@ def roofline_figure(report: dict, path: str | Path) -> None:
     plt.close(fig)
 
 
+def scaling_figure(report: dict, path: str | Path) -> None:
+    """Flips/ns against the number of spins for every variant, 2D and 3D side by side, with the
+    copy-bandwidth ceilings (figures/make_figures.py:99-124)."""
+    import matplotlib
+
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+
+    c = report["ceiling"]
+    fig, axes = plt.subplots(1, 2, figsize=(7.0, 2.9), sharey=True)
+    for ax, dim in ((axes[0], 2), (axes[1], 3)):
+        series: dict = {}
+        for r in report["runs"]:
+            if r["dim"] == dim:
+                series.setdefault((r["variant"], r["algorithm"]), []).append(r)
+        for (var, alg), rs in sorted(series.items()):
+            rs.sort(key=lambda r: r["size"][0])
+            ls = ":" if "eager" in var else ("--" if alg == "heatbath" else "-")
+            ax.plot([float(np.prod(r["size"])) for r in rs], [r["median"] for r in rs], ls,
+                    marker="o", ms=3, lw=1.2, label=f"hip/{var} {alg}")
+        for bpu, name in ((3.0, "int8"), (3.0 / 8.0, "msc")):
+            if dim == 3 and name == "msc":
+                continue
+            ax.axhline(c["copy_bandwidth_GBs"] / bpu, color="k", lw=0.8)
+            ax.text(1.5e6, c["copy_bandwidth_GBs"] / bpu * 1.08, f"copy ceiling, {name}",
+                    fontsize=6.5)
+        ax.set_xscale("log")
+        ax.set_yscale("log")
+        ax.set_xlabel(f"spins $N = L^{dim}$")
+        ax.set_title(f"{dim}D", fontsize=9)
+        ax.legend(fontsize=6, frameon=False, loc="lower right")
+    axes[0].set_ylabel("flips / ns (median)")
+    fig.tight_layout()
+    Path(path).parent.mkdir(parents=True, exist_ok=True)
+    fig.savefig(path, dpi=150)
+    plt.close(fig)
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(prog="python -m pyising_amd.report", description=__doc__,
                                  formatter_class=argparse.RawTextHelpFormatter)
@@ -332,16 +370,24 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--only", default=None, help="comma list of variants (or 'hip')")
     ap.add_argument("--markdown", default=None, help="also write the runs as a markdown table")
     ap.add_argument("--figure", default=None, help="also draw the roofline figure (png/pdf)")
+    ap.add_argument("--scaling-figure", default=None, help="also draw flips/ns against size")
+    ap.add_argument("--from-json", default=None,
+                    help="redraw the table and figures from an existing report (no GPU)")
     args = ap.parse_args(argv)
     mode = "quick" if args.quick else ("dev" if args.dev else "full")
     name = {"quick": "mi355x-quick.json", "dev": "provisional.json", "full": "mi355x.json"}[mode]
     out = Path(args.out) if args.out else Path("results") / name
     only = set(args.only.split(",")) if args.only else None
-    res = run(mode, out, args.repeats, only, log=lambda s: print(s, flush=True))
+    if args.from_json:
+        res = json.loads(Path(args.from_json).read_text())
+    else:
+        res = run(mode, out, args.repeats, only, log=lambda s: print(s, flush=True))
     if args.markdown:
         Path(args.markdown).write_text(markdown(res))
     if args.figure:
         roofline_figure(res, args.figure)
+    if args.scaling_figure:
+        scaling_figure(res, args.scaling_figure)
     return 0
 
 

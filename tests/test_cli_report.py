@@ -111,6 +111,17 @@ def test_best_per_variant_markdown_and_figure(tmp_path):
     assert len(rows) == 1 + 1 + 4  # header, rule, four runs
     report.roofline_figure(rep, tmp_path / "roofline.png")
     assert (tmp_path / "roofline.png").stat().st_size > 1000
+    report.scaling_figure(rep, tmp_path / "scaling.png")
+    assert (tmp_path / "scaling.png").stat().st_size > 1000
+
+
+def test_report_redraws_from_json(tmp_path):
+    path = tmp_path / "r.json"
+    path.write_text(json.dumps(_fake_report()))
+    assert report.main(["--quick", "--from-json", str(path), "--markdown", str(tmp_path / "r.md"),
+                        "--scaling-figure", str(tmp_path / "s.png")]) == 0
+    assert "| msc | metropolis | 8192x8192 |" in (tmp_path / "r.md").read_text()
+    assert (tmp_path / "s.png").exists()
 
 
 def _fake_validation(e_z=1.0, m=0.8, tc2=TC_2D * 1.005, tc3=TC_3D, same=True):
