@@ -26,6 +26,7 @@
 // the counter stream.
 #include <hip/hip_cooperative_groups.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -60,9 +61,14 @@ struct BatchAcc {
     unsigned long long sum_abs_m, sum_m2, sum_m4_lo, sum_m4_hi, samples;
 };
 
+constexpr u64 BATCH_NO_TILE = ~0ull;
+
 struct BatchKernelArgs {
     const BatchDesc* descs;
     const u32* tile_lat;
+    // launch form: workgroup b runs tile (u32)tile_seq[b] of lattice tile_seq[b] >> 32 (one load
+    // names both, so the descriptor load is the only dependent one); BATCH_NO_TILE: none
+    const u64* tile_seq;
     u32* plus;  // [2 sweep parity][2 colour][total_tiles]: +1 spins written by each tile
     BatchAcc* acc;
     // [0]: sweeps done, read by colour-0 work; [1]: the sweep in progress, written by colour-0
@@ -70,46 +76,58 @@ struct BatchKernelArgs {
     // concurrent workgroups of the same launch do not read)
     unsigned long long* sweeps_done;
     u32 n_lat, total_tiles, n_sweeps, msc_mode, msc_cap;
+    u32 moments;  // launch form: 0 = nobody reads the per-tile counts, skip storing them
 };
 
-// Whole-workgroup sum of v stored to *dst by thread 0 (no atomics: each tile owns its slot, so
-// there is no contention on a per-lattice counter).
-__device__ __forceinline__ void block_store_sum(u32* dst, u32 v)
+// Per-wave sum of v stored to the wave's own slot, dst[wave] (four slots per tile): no atomics
+// and no workgroup barrier, so a wave that is done retires without waiting for the others.
+constexpr u32 BATCH_WAVES = 4;  // 256-thread workgroups
+
+__device__ __forceinline__ void wave_store_sum(u32* dst, u32 v)
 {
-    __shared__ u32 red[4];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) *dst = red[0] + red[1] + red[2] + red[3];
-    __syncthreads();
+    if ((threadIdx.x & 63) == 0) dst[threadIdx.x >> 6] = v;
 }
 
 // One wave: M of lattice `lat` from its tiles' counts of parity `par`, folded into the
 // accumulators (sum M, sum |M|, sum M^2, 128-bit sum M^4, samples).
+// The +1 spins of lattice `lat` in sweep parity `par`: this thread's share of its count
+// slots, threads `first`, `first + step`, ...
+__device__ __forceinline__ u64 batch_lattice_plus(const BatchKernelArgs& ka, const BatchDesc& d,
+                                                  u32 par, u32 first, u32 step)
+{
+    const size_t slots = (size_t)BATCH_WAVES * ka.total_tiles;  // per (parity, colour)
+    const u32* p0 = ka.plus + ((size_t)par * 2 + 0) * slots + (size_t)BATCH_WAVES * d.tile_begin;
+    const u32* p1 = ka.plus + ((size_t)par * 2 + 1) * slots + (size_t)BATCH_WAVES * d.tile_begin;
+    u64 plus = 0;
+    for (u32 i = first; i < BATCH_WAVES * d.n_tiles; i += step) plus += (u64)p0[i] + p1[i];
+    return plus;
+}
+
+__device__ __forceinline__ void batch_fold_moments(BatchAcc& A, const BatchDesc& d, u64 plus)
+{
+    const long long m = 2 * (long long)plus - d.n_spins;
+    const u64 m2 = (u64)(m * m);
+    const u64 lo = m2 * m2, hi = __umul64hi(m2, m2);
+    A.sum_m += m;
+    A.sum_abs_m += (u64)(m < 0 ? -m : m);
+    A.sum_m2 += m2;
+    const u64 nlo = A.sum_m4_lo + lo;
+    A.sum_m4_hi += hi + (nlo < lo ? 1u : 0u);
+    A.sum_m4_lo = nlo;
+    A.samples += 1;
+}
+
 __device__ __forceinline__ void batch_accumulate_lattice(const BatchKernelArgs& ka, u32 lat, u32 par)
 {
     const BatchDesc& d = ka.descs[lat];
-    const u32* p0 = ka.plus + ((size_t)par * 2 + 0) * ka.total_tiles + d.tile_begin;
-    const u32* p1 = ka.plus + ((size_t)par * 2 + 1) * ka.total_tiles + d.tile_begin;
-    u64 plus = 0;
-    for (u32 i = threadIdx.x & 63u; i < d.n_tiles; i += 64u) plus += (u64)p0[i] + p1[i];
+    u64 plus = batch_lattice_plus(ka, d, par, threadIdx.x & 63u, 64u);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) plus += __shfl_xor(plus, off, 64);
-    if ((threadIdx.x & 63) == 0) {
-        const long long m = 2 * (long long)plus - d.n_spins;
-        BatchAcc& A = ka.acc[lat];
-        const u64 m2 = (u64)(m * m);
-        const u64 lo = m2 * m2, hi = __umul64hi(m2, m2);
-        A.sum_m += m;
-        A.sum_abs_m += (u64)(m < 0 ? -m : m);
-        A.sum_m2 += m2;
-        const u64 nlo = A.sum_m4_lo + lo;
-        A.sum_m4_hi += hi + (nlo < lo ? 1u : 0u);
-        A.sum_m4_lo = nlo;
-        A.samples += 1;
-    }
+    if ((threadIdx.x & 63) == 0) batch_fold_moments(ka.acc[lat], d, plus);
 }
+
 
 // One 512-word msc tile by queue compaction (4 unrolled calls, then packed rounds), with the
 // lattice's wave classification (block-uniform branch).
@@ -160,11 +178,12 @@ __device__ __forceinline__ u32 batch_int8_tile(const BatchDesc& d, const u32* th
     return (u32)V - minus;
 }
 
-template <int NDIM, int HB>
+// REUSE: the workgroup ran an earlier tile whose threads may still read thr_lds.
+template <int NDIM, int HB, bool REUSE>
 __device__ __forceinline__ u32 batch_int8_v(const BatchDesc& d, u32* thr_lds, u32 local, u32 t,
                                             u32 color)
 {
-    __syncthreads();  // the previous tile's readers of thr_lds are done
+    if (REUSE) __syncthreads();
     int8_fill_table<NDIM, HB>(thr_lds, d.tb);
     __syncthreads();
     if (d.vec == 16) return batch_int8_tile<NDIM, HB, 16>(d, thr_lds, local, t, color);
@@ -191,10 +210,10 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
                 const u32 t = d.t0 + sw;
                 u32 plus = 0;
                 if (d.kind == BATCH_INT8) {
-                    if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1>(d, thr_lds, local, t, color)
-                                                 : batch_int8_v<2, 0>(d, thr_lds, local, t, color);
-                    else plus = d.hb ? batch_int8_v<3, 1>(d, thr_lds, local, t, color)
-                                     : batch_int8_v<3, 0>(d, thr_lds, local, t, color);
+                    if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1, true>(d, thr_lds, local, t, color)
+                                                 : batch_int8_v<2, 0, true>(d, thr_lds, local, t, color);
+                    else plus = d.hb ? batch_int8_v<3, 1, true>(d, thr_lds, local, t, color)
+                                     : batch_int8_v<3, 0, true>(d, thr_lds, local, t, color);
                 } else {
                     MscArgs a;
                     a.own = (u64*)(color ? d.c1 : d.c0);
@@ -210,7 +229,8 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
                     plus = d.msc_queue ? batch_msc_tile(a, d.codes, local * 512u, d.layout, lds)
                                        : batch_msc_lockstep(a, d.codes, local, d.layout);
                 }
-                block_store_sum(ka.plus + ((size_t)par * 2 + color) * ka.total_tiles + tile, plus);
+                wave_store_sum(ka.plus + ((size_t)par * 2 + color) * BATCH_WAVES * ka.total_tiles +
+                                   (size_t)BATCH_WAVES * tile, plus);
             }
             grid.sync();
         }
@@ -232,23 +252,24 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
 {
     __shared__ MscQueueLds<2> lds;
     __shared__ u32 thr_lds[8];
-    const u32 tile = xcd_band_block(blockIdx.x, gridDim.x);
+    const u64 entry = ka.tile_seq[blockIdx.x];
     const u32 sw = (u32)ka.sweeps_done[color];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (color == 0u) ka.sweeps_done[1] = sw;
         else ka.sweeps_done[0] = sw + 1u;
     }
+    if (entry == BATCH_NO_TILE) return;  // padding of a shorter XCD list (whole workgroup)
     const u32 par = sw & 1u;
-    const u32 lat = ka.tile_lat[tile];
+    const u32 tile = (u32)entry, lat = (u32)(entry >> 32);
     const BatchDesc& d = ka.descs[lat];
     const u32 local = tile - d.tile_begin;
     const u32 t = d.t0 + sw;
     u32 plus = 0;
     if (d.kind == BATCH_INT8) {
-        if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1>(d, thr_lds, local, t, color)
-                                     : batch_int8_v<2, 0>(d, thr_lds, local, t, color);
-        else plus = d.hb ? batch_int8_v<3, 1>(d, thr_lds, local, t, color)
-                         : batch_int8_v<3, 0>(d, thr_lds, local, t, color);
+        if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1, false>(d, thr_lds, local, t, color)
+                                     : batch_int8_v<2, 0, false>(d, thr_lds, local, t, color);
+        else plus = d.hb ? batch_int8_v<3, 1, false>(d, thr_lds, local, t, color)
+                         : batch_int8_v<3, 0, false>(d, thr_lds, local, t, color);
     } else {
         MscArgs a;
         a.own = (u64*)(color ? d.c1 : d.c0);
@@ -264,16 +285,20 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
         plus = d.msc_queue ? batch_msc_tile(a, d.codes, local * 512u, d.layout, lds)
                            : batch_msc_lockstep(a, d.codes, local, d.layout);
     }
-    block_store_sum(ka.plus + ((size_t)par * 2 + color) * ka.total_tiles + tile, plus);
+    if (ka.moments)
+        wave_store_sum(ka.plus + ((size_t)par * 2 + color) * BATCH_WAVES * ka.total_tiles +
+                           (size_t)BATCH_WAVES * tile, plus);
 }
 
-// After both colours of a sweep (when moments are on): one wave per lattice, four per
-// workgroup, spread over the CUs, accumulates the sweep that the colour-1 launch just finished.
+// After both colours of a sweep (when moments are on): one workgroup per lattice, spread over
+// the CUs, accumulates the sweep that the colour-1 launch just finished.
 __global__ __launch_bounds__(256) void batch_accumulate(BatchKernelArgs ka)
 {
     const u32 sw = (u32)ka.sweeps_done[1];
-    const u32 lat = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (lat < ka.n_lat) batch_accumulate_lattice(ka, lat, sw & 1u);
+    const BatchDesc& d = ka.descs[blockIdx.x];
+    u64 plus = batch_lattice_plus(ka, d, sw & 1u, threadIdx.x, 256u), unused = 0;
+    block_sum2_u64(plus, unused);
+    if (threadIdx.x == 0) batch_fold_moments(ka.acc[blockIdx.x], d, plus);
 }
 
 __global__ void batch_zero_u64(unsigned long long* p, u32 n)
@@ -286,10 +311,11 @@ __global__ void batch_zero_u64(unsigned long long* p, u32 n)
 // ising_hip.h.
 
 struct BatchHandle {
-    u32 n_lat = 0, total_tiles = 0;
-    u32 cap_lat = 0, cap_tiles = 0;  // device table capacity (reserved handles: > n_lat)
+    u32 n_lat = 0, total_tiles = 0, n_seq = 0;
+    u32 cap_lat = 0, cap_tiles = 0, cap_seq = 0;  // device table capacity (reserved: > n_lat)
     BatchDesc* d_descs = nullptr;
     u32* d_tile_lat = nullptr;
+    u64* d_tile_seq = nullptr;
     u32* d_plus = nullptr;
     BatchAcc* d_acc = nullptr;
     unsigned long long* d_done = nullptr;
@@ -329,6 +355,7 @@ static void batch_free(BatchHandle* h)
     }
     (void)hipFree(h->d_descs);
     (void)hipFree(h->d_tile_lat);
+    (void)hipFree(h->d_tile_seq);
     (void)hipFree(h->d_plus);
     (void)hipFree(h->d_acc);
     (void)hipFree(h->d_done);
@@ -412,6 +439,56 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
     return ISING_OK;
 }
 
+// Launch order of the tiles of one half sweep. Workgroup b runs on XCD b % 8, and the XCDs
+// have their own L2s, so the schedule is eight lists interleaved: every lattice is cut into
+// eight contiguous chunks of tiles, one per list (a chunk's halo rows stay in its XCD's L2 and
+// every XCD gets the same share of every lattice, so their work is equal whatever the mix);
+// within a list, lattices with longer tiles come first (msc compaction, then msc lockstep,
+// then int8), so the last workgroups to start are short ones. Shorter lists are padded with
+// BATCH_NO_TILE. The order changes no result: a tile computes the same words wherever it runs.
+static u32 batch_tile_cost(const BatchDesc& d)
+{
+    if (d.kind == BATCH_INT8) return 1u;
+    return d.msc_queue ? 3u : 2u;
+}
+
+static void batch_schedule(const std::vector<BatchDesc>& descs, std::vector<u64>& seq)
+{
+    constexpr u32 X = 8;
+    std::vector<u32> order(descs.size());
+    for (u32 i = 0; i < (u32)descs.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+        return batch_tile_cost(descs[a]) > batch_tile_cost(descs[b]);
+    });
+    std::vector<u64> lists[X];
+    for (u32 li : order) {
+        const BatchDesc& d = descs[li];
+        const u32 q = d.n_tiles / X, r = d.n_tiles % X;
+        // lists by length, shortest first: they take the r chunks of q + 1 tiles
+        u32 by_len[X];
+        for (u32 x = 0; x < X; ++x) by_len[x] = x;
+        std::stable_sort(by_len, by_len + X, [&](u32 a, u32 b) { return lists[a].size() < lists[b].size(); });
+        u32 next = d.tile_begin;
+        for (u32 k = 0; k < X; ++k) {
+            const u32 len = q + (k < r ? 1u : 0u);
+            for (u32 t = 0; t < len; ++t) lists[by_len[k]].push_back(((u64)li << 32) | (next + t));
+            next += len;
+        }
+    }
+    size_t longest = 0;
+    for (u32 x = 0; x < X; ++x) longest = lists[x].size() > longest ? lists[x].size() : longest;
+    seq.assign(longest * X, BATCH_NO_TILE);
+    for (u32 x = 0; x < X; ++x)
+        for (size_t k = 0; k < lists[x].size(); ++k) seq[k * X + x] = lists[x][k];
+}
+
+// Capacity of the launch order for cap_lat lattices and cap_tiles tiles: each lattice can
+// lengthen the longest list by at most one tile beyond its even share.
+static u32 batch_seq_capacity(u32 cap_lat, u32 cap_tiles)
+{
+    return cap_tiles + 8u * (cap_lat + 1u);
+}
+
 // Allocates the device tables for cap_lat lattices and cap_tiles tiles and sets the grid
 // parameters (synchronous).
 static int batch_alloc(BatchHandle* h, u32 cap_lat, u32 cap_tiles, int blocks_per_cu)
@@ -424,18 +501,21 @@ static int batch_alloc(BatchHandle* h, u32 cap_lat, u32 cap_tiles, int blocks_pe
     h->per_cu = (blocks_per_cu <= 0 || blocks_per_cu > h->max_blocks) ? h->max_blocks : blocks_per_cu;
     h->cap_lat = cap_lat;
     h->cap_tiles = cap_tiles;
+    h->cap_seq = batch_seq_capacity(cap_lat, cap_tiles);
     ISING_HIP(hipMalloc((void**)&h->d_descs, sizeof(BatchDesc) * (size_t)cap_lat));
     ISING_HIP(hipMalloc((void**)&h->d_tile_lat, sizeof(u32) * (size_t)cap_tiles));
-    ISING_HIP(hipMalloc((void**)&h->d_plus, sizeof(u32) * 4 * (size_t)cap_tiles));
+    ISING_HIP(hipMalloc((void**)&h->d_tile_seq, sizeof(u64) * (size_t)h->cap_seq));
+    ISING_HIP(hipMalloc((void**)&h->d_plus, sizeof(u32) * 4 * BATCH_WAVES * (size_t)cap_tiles));
     ISING_HIP(hipMalloc((void**)&h->d_acc, sizeof(BatchAcc) * (size_t)cap_lat));
     ISING_HIP(hipMalloc((void**)&h->d_done, 2 * sizeof(unsigned long long)));
     return ISING_OK;
 }
 
-static void batch_set_sizes(BatchHandle* h, u32 n_lat, u32 tiles)
+static void batch_set_sizes(BatchHandle* h, u32 n_lat, u32 tiles, u32 n_seq)
 {
     h->n_lat = n_lat;
     h->total_tiles = tiles;
+    h->n_seq = n_seq;
     h->grid = h->per_cu * h->n_cu;
     if ((u32)h->grid > tiles) h->grid = (int)tiles;
 }
@@ -458,7 +538,9 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
     BatchKernelArgs ka;
     ka.descs = h->d_descs;
     ka.tile_lat = h->d_tile_lat;
+    ka.tile_seq = h->d_tile_seq;
     ka.plus = h->d_plus;
+    ka.moments = (u32)h->moments;
     ka.acc = h->d_acc;
     ka.sweeps_done = h->d_done;
     ka.n_lat = h->n_lat;
@@ -473,10 +555,10 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
         return ISING_OK;
     }
     for (u32 s = 0; s < n_sweeps; ++s) {
-        hipLaunchKernelGGL(batch_half, dim3(h->total_tiles), dim3(256), 0, st, ka, 0u);
-        hipLaunchKernelGGL(batch_half, dim3(h->total_tiles), dim3(256), 0, st, ka, 1u);
+        hipLaunchKernelGGL(batch_half, dim3(h->n_seq), dim3(256), 0, st, ka, 0u);
+        hipLaunchKernelGGL(batch_half, dim3(h->n_seq), dim3(256), 0, st, ka, 1u);
         if (h->moments)
-            hipLaunchKernelGGL(batch_accumulate, dim3((h->n_lat + 3u) / 4u), dim3(256), 0, st, ka);
+            hipLaunchKernelGGL(batch_accumulate, dim3(h->n_lat), dim3(256), 0, st, ka);
     }
     ISING_LAUNCHED("batch_half");
     return ISING_OK;
@@ -501,6 +583,8 @@ int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, 
     std::vector<u32> tile_lat;
     if (int rc = batch_build(lats, n, (u32)msc_mode, descs, tile_lat)) return rc;
     const u32 tiles = (u32)tile_lat.size();
+    std::vector<u64> seq;
+    batch_schedule(descs, seq);
 
     BatchHandle* h = new BatchHandle();
     h->msc_mode = (u32)msc_mode;
@@ -510,9 +594,10 @@ int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, 
         batch_free(h);
         return rc;
     }
-    batch_set_sizes(h, (u32)n, tiles);
+    batch_set_sizes(h, (u32)n, tiles, (u32)seq.size());
     hipError_t e = hipMemcpy(h->d_descs, descs.data(), sizeof(BatchDesc) * (size_t)n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_tile_lat, tile_lat.data(), sizeof(u32) * tile_lat.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_tile_seq, seq.data(), sizeof(u64) * seq.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) rc = batch_zero_stats(h, 0);
     if (e == hipSuccess && !rc) e = hipDeviceSynchronize();
     if (e != hipSuccess || rc) {
@@ -549,13 +634,14 @@ int ising_batch_reserve(int max_lattices, long long max_tiles, int blocks_per_cu
     h->msc_mode = (u32)msc_mode;
     h->msc_cap = (u32)msc_cap;
     int rc = batch_alloc(h, (u32)max_lattices, (u32)max_tiles, blocks_per_cu);
-    const size_t bytes = sizeof(BatchDesc) * (size_t)max_lattices + sizeof(u32) * (size_t)max_tiles;
+    const size_t bytes = sizeof(BatchDesc) * (size_t)max_lattices + sizeof(u32) * (size_t)max_tiles +
+                         8 + sizeof(u64) * (size_t)h->cap_seq;
     for (int s = 0; s < 2 && !rc; ++s) {
         hipError_t e = hipHostMalloc(&h->pinned[s], bytes, hipHostMallocDefault);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&h->pinned_done[s], hipEventDisableTiming);
         if (e != hipSuccess) rc = ising_check_hip(e, "batch_reserve");
     }
-    batch_set_sizes(h, 0, 0);
+    batch_set_sizes(h, 0, 0, 0);
     if (!rc) rc = batch_zero_stats(h, 0);
     if (!rc) rc = ising_check_hip(hipDeviceSynchronize(), "batch_reserve");
     if (rc) {
@@ -581,6 +667,9 @@ int ising_batch_assign(void* handle, const IsingBatchLattice* lats, int n, void*
     std::vector<u32> tile_lat;
     if (int rc = batch_build(lats, n, h->msc_mode, descs, tile_lat)) return rc;
     if (tile_lat.size() > h->cap_tiles) return ising_fail(ISING_ERANGE, "batch_assign: more tiles than reserved");
+    std::vector<u64> seq;
+    batch_schedule(descs, seq);
+    if (seq.size() > h->cap_seq) return ising_fail(ISING_ERANGE, "batch_assign: launch order exceeds its capacity");
     hipStream_t st = as_stream(stream);
     const int s = h->next_slot;
     h->next_slot ^= 1;
@@ -595,10 +684,14 @@ int ising_batch_assign(void* handle, const IsingBatchLattice* lats, int n, void*
     if (map_bytes) {
         memcpy(stage + desc_bytes, tile_lat.data(), map_bytes);
         ISING_HIP(hipMemcpyAsync(h->d_tile_lat, stage + desc_bytes, map_bytes, hipMemcpyHostToDevice, st));
+        const size_t seq_bytes = sizeof(u64) * seq.size();
+        const size_t seq_at = (desc_bytes + map_bytes + 7u) & ~(size_t)7u;
+        memcpy(stage + seq_at, seq.data(), seq_bytes);
+        ISING_HIP(hipMemcpyAsync(h->d_tile_seq, stage + seq_at, seq_bytes, hipMemcpyHostToDevice, st));
     }
     ISING_HIP(hipEventRecord(h->pinned_done[s], st));
     batch_drop_graph(h);
-    batch_set_sizes(h, (u32)n, (u32)tile_lat.size());
+    batch_set_sizes(h, (u32)n, (u32)tile_lat.size(), (u32)seq.size());
     return batch_zero_stats(h, st);
 }
 
