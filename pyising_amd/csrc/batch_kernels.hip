@@ -77,6 +77,7 @@ struct BatchKernelArgs {
     unsigned long long* sweeps_done;
     u32 n_lat, total_tiles, n_sweeps, msc_mode, msc_cap;
     u32 moments;  // launch form: 0 = nobody reads the per-tile counts, skip storing them
+    u32 n_seq;    // launch form: tile workgroups per half sweep (the launch order's length)
 };
 
 // Per-wave sum of v stored to the wave's own slot, dst[wave] (four slots per tile): no atomics
@@ -248,16 +249,27 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
 // the barrier between colours. The sweep index lives in device memory, so captured graphs
 // replay with it advancing: a colour-0 launch reads sweeps_done[0] and copies it to [1], the
 // colour-1 launch reads [1] and sets [0] to the next sweep.
-__global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color)
+// acc_prev (colour 0 only): the n_lat workgroups after the n_seq tile workgroups accumulate the
+// moments of the previous sweep, whose count slots (the other parity) this sweep does not write;
+// this saves the accumulation launch of every sweep but a call's last.
+__global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color, u32 acc_prev)
 {
     __shared__ MscQueueLds<2> lds;
     __shared__ u32 thr_lds[8];
-    const u64 entry = ka.tile_seq[blockIdx.x];
     const u32 sw = (u32)ka.sweeps_done[color];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (color == 0u) ka.sweeps_done[1] = sw;
         else ka.sweeps_done[0] = sw + 1u;
     }
+    if (acc_prev && blockIdx.x >= ka.n_seq) {
+        const u32 lat = blockIdx.x - ka.n_seq;
+        const BatchDesc& dl = ka.descs[lat];
+        u64 plus = batch_lattice_plus(ka, dl, (sw - 1u) & 1u, threadIdx.x, 256u), unused = 0;
+        block_sum2_u64(plus, unused);
+        if (threadIdx.x == 0) batch_fold_moments(ka.acc[lat], dl, plus);
+        return;
+    }
+    const u64 entry = ka.tile_seq[blockIdx.x];
     if (entry == BATCH_NO_TILE) return;  // padding of a shorter XCD list (whole workgroup)
     const u32 par = sw & 1u;
     const u32 tile = (u32)entry, lat = (u32)(entry >> 32);
@@ -541,6 +553,7 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
     ka.tile_seq = h->d_tile_seq;
     ka.plus = h->d_plus;
     ka.moments = (u32)h->moments;
+    ka.n_seq = h->n_seq;
     ka.acc = h->d_acc;
     ka.sweeps_done = h->d_done;
     ka.n_lat = h->n_lat;
@@ -555,11 +568,13 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
         return ISING_OK;
     }
     for (u32 s = 0; s < n_sweeps; ++s) {
-        hipLaunchKernelGGL(batch_half, dim3(h->n_seq), dim3(256), 0, st, ka, 0u);
-        hipLaunchKernelGGL(batch_half, dim3(h->n_seq), dim3(256), 0, st, ka, 1u);
-        if (h->moments)
-            hipLaunchKernelGGL(batch_accumulate, dim3(h->n_lat), dim3(256), 0, st, ka);
+        const u32 acc_prev = (h->moments && s > 0) ? 1u : 0u;
+        hipLaunchKernelGGL(batch_half, dim3(h->n_seq + (acc_prev ? h->n_lat : 0u)), dim3(256), 0,
+                           st, ka, 0u, acc_prev);
+        hipLaunchKernelGGL(batch_half, dim3(h->n_seq), dim3(256), 0, st, ka, 1u, 0u);
     }
+    // the last sweep's moments (earlier ones were folded into the next colour-0 launch)
+    if (h->moments) hipLaunchKernelGGL(batch_accumulate, dim3(h->n_lat), dim3(256), 0, st, ka);
     ISING_LAUNCHED("batch_half");
     return ISING_OK;
 }
