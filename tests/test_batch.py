@@ -64,6 +64,30 @@ def test_batch_bitwise_equals_standalone(hip, msc_mode, persistent):
 
 
 @pytest.mark.gpu
+def test_persistent_tiny_grid_orders_the_colours(hip):
+    """The persistent form with one workgroup per CU: every workgroup runs several tiles per
+    half sweep, so colour 1 of a lattice is correct only if the grid barrier really waits for
+    every colour-0 tile (the race a checkerboard kernel can have)."""
+    from pyising_amd.batch import BatchScan
+
+    specs = [(hip.HipInt8, (2048, 2048), 1 / 2.269, "metropolis"),
+             (hip.HipMultiSpin, (4096, 4096), 1 / 2.269, "metropolis"),
+             (hip.HipInt8, (64, 64, 64), 0.2216, "heatbath")]
+    sims = [cls(sh, b, seed=31 + i, algorithm=al) for i, (cls, sh, b, al) in enumerate(specs)]
+    refs = [cls(sh, b, seed=31 + i, algorithm=al) for i, (cls, sh, b, al) in enumerate(specs)]
+    scan = BatchScan(sims, persistent=True, blocks_per_cu=1)
+    info = scan.info()
+    assert info["tiles_per_half_sweep"] > 2 * info["grid"]
+    scan.sweep(3)
+    scan.synchronize()
+    for sim, ref in zip(sims, refs, strict=True):
+        ref.sweep(3)
+        assert sim.observables_exact() == ref.observables_exact()
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    scan.close()
+
+
+@pytest.mark.gpu
 def test_batch_matches_oracle(hip):
     from pyising_amd.batch import BatchScan
 
