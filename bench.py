@@ -58,10 +58,40 @@ def cpu_baseline(seconds: float = 10.0) -> dict:
         if el >= seconds:
             break
     rate = sweeps * L_HEADLINE**2 / el / 1e9
-    return {"value": round(rate, 4), "unit": "flips/ns", "cores": 1, "kind": "port",
-            "sample": f"oracle NumPy msc sweep, L={L_HEADLINE} T_c seed 1 hot start, "
-                      f"{sweeps} sweeps in {el:.1f} s on one core of {cpu_model()} "
-                      f"(os.cpu_count()={os.cpu_count()})"}
+    out = {"value": round(rate, 4), "unit": "flips/ns", "cores": 1, "kind": "port",
+           "sample": f"oracle NumPy msc sweep, L={L_HEADLINE} T_c seed 1 hot start, "
+                     f"{sweeps} sweeps in {el:.1f} s on one core of {cpu_model()} "
+                     f"(os.cpu_count()={os.cpu_count()})"}
+    try:
+        out["openmp_c"] = cpu_baseline_openmp(min(seconds, 5.0))
+    except (OSError, RuntimeError) as exc:  # the C restatement was not built
+        out["openmp_c"] = {"error": str(exc)[:200]}
+    return out
+
+
+def cpu_baseline_openmp(seconds: float = 10.0, threads: int = 16) -> dict:
+    """The oracle's C restatement of the same msc sweep (same lattice and stream) with OpenMP
+    over rows on ``threads`` host threads (the box's CPU share), timed like the NumPy one."""
+    from oracle import c_oracle as CO
+    from pyising_amd.lattice import BETA_C_2D
+
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    CO.set_threads(threads)
+    sim = CO.CMsc((L_HEADLINE, L_HEADLINE), BETA_C_2D, seed=1)
+    sim.sweep(1)  # warm-up
+    sweeps = 0
+    t0 = time.perf_counter()
+    while True:
+        sim.sweep(1)
+        sweeps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    rate = sweeps * L_HEADLINE**2 / el / 1e9
+    return {"value": round(rate, 4), "unit": "flips/ns", "cores": threads, "kind": "port",
+            "sample": f"oracle C msc sweep (OpenMP over rows), L={L_HEADLINE} T_c seed 1 hot "
+                      f"start, {sweeps} sweeps in {el:.1f} s on {threads} threads of "
+                      f"{cpu_model()}"}
 
 
 def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
