@@ -143,6 +143,50 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
             "binder_L4096_Tmid": round(mom[-9].binder, 4)}
 
 
+def workloads() -> dict:
+    """The other single-lattice workloads, timed end to end on their own streams (measurements
+    included): 2D int8 L=1024 at beta=0.4406868, seed 1, hot, 100 sweeps then (M, E); 2D int8
+    L=8192 at T in {1.5, 2.269, 3.0}, 200 sweeps each; 3D int8 L=256 at K_c, seed 7, both
+    algorithms, hot, 2000 sweeps with (E, |M|) every 10 sweeps."""
+    from pyising_amd import _lib
+    from pyising_amd.backends.hip_backend import HipInt8
+    from pyising_amd.lattice import BETA_C_3D
+
+    out = {}
+    stream = _lib.Stream()
+
+    def timed(sim, sweeps, every=None):
+        sim.sweep(20)  # clocks up
+        sim.synchronize()
+        series = []
+        t0 = time.perf_counter()
+        if every:  # exact (M, E) gathered on the device, read back once
+            series = [tuple(x) for x in sim.measure_series(sweeps // every, every).tolist()]
+        else:
+            sim.sweep(sweeps)
+            series.append(sim.observables_exact())
+        dt = time.perf_counter() - t0
+        return sweeps * sim.n_spins / dt / 1e9, dt, series
+
+    sim = HipInt8((1024, 1024), 0.4406868, seed=1, stream=stream.ptr)
+    rate, dt, ser = timed(sim, 100)
+    out["int8_2d_L1024_100_sweeps"] = {"flips_per_ns": round(rate, 1), "ms": round(dt * 1e3, 3),
+                                       "M_E_after": list(ser[-1])}
+    for T in (1.5, 2.269, 3.0):
+        sim = HipInt8((8192, 8192), 1.0 / T, seed=1, stream=stream.ptr)
+        rate, dt, ser = timed(sim, 200)
+        out[f"int8_2d_L8192_T{T}"] = {"flips_per_ns": round(rate, 1), "M_E_after": list(ser[-1])}
+    for alg in ("metropolis", "heatbath"):
+        sim = HipInt8((256, 256, 256), BETA_C_3D, seed=7, algorithm=alg, stream=stream.ptr)
+        rate, dt, ser = timed(sim, 2000, every=10)
+        out[f"int8_3d_L256_Kc_{alg}_2000_sweeps_obs_every_10"] = {
+            "flips_per_ns": round(rate, 1), "ms": round(dt * 1e3, 1),
+            "mean_abs_m": round(float(np.mean([abs(m) for m, _ in ser[len(ser) // 2:]])) / 256**3, 5),
+            "mean_e": round(float(np.mean([e for _, e in ser[len(ser) // 2:]])) / 256**3, 5)}
+        del sim
+    return out
+
+
 def serving(n_requests: int = 300, rate: float = 200.0, seed: int = 0) -> dict:
     """The serving loop on the Poisson request stream: arrivals at ``rate`` per second
     (2D L = 32..4096 int8/msc with p = 0.7, else 3D L = 16..256 int8 Metropolis/heat bath;
@@ -181,6 +225,7 @@ def main() -> int:
     ap.add_argument("--no-ceilings", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path "
@@ -269,6 +314,8 @@ def main() -> int:
         }
     if not args.no_batch:
         result["batched_scan"] = batch_vs_plain()
+    if not args.no_workloads:
+        result["workloads"] = workloads()
     if not args.no_serve:
         result["serving"] = serving()
     if not args.no_cpu_baseline:

@@ -102,6 +102,24 @@ class _HipBase(IsingSimulation):
     def synchronize(self) -> None:
         _lib.check(self._L.ising_stream_sync(self.stream_ptr), "stream_sync")
 
+    def measure_series(self, n_meas: int, every: int = 1) -> np.ndarray:
+        """``n_meas`` exact ``(M, E)`` pairs, one after every ``every`` sweeps, as an
+        ``(n_meas, 2)`` int64 array: the same numbers as calling ``sweep(every)`` then
+        ``observables_exact()`` in a loop, but each pair goes to its own device slot and all
+        are read back once at the end, so the host never waits between measurements."""
+        n_meas, every = int(n_meas), int(every)
+        out = np.zeros((max(n_meas, 0), 2), dtype=np.int64)
+        if n_meas <= 0:
+            return out
+        buf = _lib.DeviceBuffer(16 * n_meas)
+        for i in range(n_meas):
+            self.sweep(every)
+            self._obs_into(buf.ptr + 16 * i)
+        buf.download(out, self.stream_ptr)
+        self.synchronize()
+        buf.free()
+        return out
+
     def _read_obs(self) -> tuple[int, int]:
         out = np.zeros(2, dtype=np.int64)
         self._obs_out.download(out, self.stream_ptr)
@@ -171,10 +189,13 @@ class HipInt8(_HipBase):
             self.t & 0xFFFFFFFF, n, self.k0, self.k1, self.stream_ptr), "int8_sweeps")
         self.t += n
 
-    def observables_exact(self) -> tuple[int, int]:
+    def _obs_into(self, out_ptr) -> None:
         _lib.check(self._L.ising_int8_obs(
             self.c[0].ptr, self.c[1].ptr, self.rows, self.lh, self.ly, self.lz, self.ndim,
-            self._obs_scratch.ptr, self._obs_out.ptr, self.stream_ptr), "int8_obs")
+            self._obs_scratch.ptr, out_ptr, self.stream_ptr), "int8_obs")
+
+    def observables_exact(self) -> tuple[int, int]:
+        self._obs_into(self._obs_out.ptr)
         return self._read_obs()
 
 
@@ -271,10 +292,13 @@ class HipMultiSpin(_HipBase):
         self._run(n)
         self.t += n
 
-    def observables_exact(self) -> tuple[int, int]:
+    def _obs_into(self, out_ptr) -> None:
         _lib.check(self._L.ising_msc_obs(self.w[0].ptr, self.w[1].ptr, self.rows, self.nw,
-                                         self._obs_scratch.ptr, self._obs_out.ptr,
-                                         self.stream_ptr), "msc_obs")
+                                         self._obs_scratch.ptr, out_ptr, self.stream_ptr),
+                   "msc_obs")
+
+    def observables_exact(self) -> tuple[int, int]:
+        self._obs_into(self._obs_out.ptr)
         return self._read_obs()
 
     def philox_stats(self, sweeps: int = 2) -> dict:

@@ -91,8 +91,14 @@ def jackknife_binder_error(m: np.ndarray, n_bins: int = 32) -> float:
 def sample(sim, n_therm: int, n_meas: int, every: int = 1) -> Series:
     """Thermalise for ``n_therm`` sweeps, then take ``n_meas`` measurements ``every`` sweeps
     apart (src/pyising/observables.py:62-75). Any object with ``sweep(n)`` and either
-    ``observables()`` or ``magnetization()``/``energy()`` works."""
+    ``observables()`` or ``magnetization()``/``energy()`` works; one with
+    ``measure_series(n, every)`` (the hip simulations: exact (M, E) pairs gathered on the
+    device) is asked for the whole series at once."""
     sim.sweep(int(n_therm))
+    series = getattr(sim, "measure_series", None)
+    if series is not None:  # device-side series: same numbers, no host wait per measurement
+        me = series(int(n_meas), int(every)).astype(float) / sim.n_spins
+        return Series(me[:, 0].copy(), me[:, 1].copy())
     m = np.empty(int(n_meas))
     e = np.empty(int(n_meas))
     measure = getattr(sim, "observables", None)

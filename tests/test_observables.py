@@ -74,6 +74,44 @@ def test_sample_and_summary():
     assert summ["n"] == 5 and summ["abs_m"] == pytest.approx(0.19)
 
 
+class _FakeSeries(_Fake):
+    """The same 'simulation' offering a device-side series of exact (M, E) over 100 spins."""
+
+    n_spins = 100
+
+    def measure_series(self, n, every):
+        out = []
+        for _ in range(n):
+            self.sweep(every)
+            out.append((self.t, -2 * self.t))
+        return np.array(out, dtype=np.int64)
+
+
+def test_sample_uses_the_device_series():
+    a = obs.sample(_Fake(), n_therm=10, n_meas=5, every=3)
+    b = obs.sample(_FakeSeries(), n_therm=10, n_meas=5, every=3)
+    np.testing.assert_allclose(b.m, a.m)
+    np.testing.assert_allclose(b.e, a.e)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,shape", [("int8", (64, 64)), ("int8", (16, 16, 16)),
+                                           ("msc", (128, 256))])
+def test_measure_series_equals_the_host_loop(variant, shape):
+    from pyising_amd.backends.hip_backend import HipInt8, HipMultiSpin
+
+    cls = HipInt8 if variant == "int8" else HipMultiSpin
+    beta = 0.22 if len(shape) == 3 else 0.44
+    a, b = cls(shape, beta, seed=5), cls(shape, beta, seed=5)
+    ser = a.measure_series(7, every=3)
+    loop = []
+    for _ in range(7):
+        b.sweep(3)
+        loop.append(b.observables_exact())
+    np.testing.assert_array_equal(ser, np.array(loop, dtype=np.int64))
+    assert a.t == b.t == 21
+
+
 @pytest.mark.gpu
 def test_scan_binder_equals_time_series():
     from pyising_amd.backends.hip_backend import HipInt8, HipMultiSpin
