@@ -350,7 +350,8 @@ def scaling_figure(report: dict, path: str | Path) -> None:
         ax.set_yscale("log")
         ax.set_xlabel(f"spins $N = L^{dim}$")
         ax.set_title(f"{dim}D", fontsize=9)
-        ax.legend(fontsize=6, frameon=False, loc="lower right")
+        if series:
+            ax.legend(fontsize=6, frameon=False, loc="lower right")
     axes[0].set_ylabel("flips / ns (median)")
     fig.tight_layout()
     Path(path).parent.mkdir(parents=True, exist_ok=True)
