@@ -344,8 +344,8 @@ def scaling_figure(report: dict, path: str | Path) -> None:
             if dim == 3 and name == "msc":
                 continue
             ax.axhline(c["copy_bandwidth_GBs"] / bpu, color="k", lw=0.8)
-            ax.text(1.5e6, c["copy_bandwidth_GBs"] / bpu * 1.08, f"copy ceiling, {name}",
-                    fontsize=6.5)
+            ax.text(0.02, c["copy_bandwidth_GBs"] / bpu * 1.08, f"copy ceiling, {name}",
+                    fontsize=6.5, transform=ax.get_yaxis_transform())
         ax.set_xscale("log")
         ax.set_yscale("log")
         ax.set_xlabel(f"spins $N = L^{dim}$")
