@@ -33,6 +33,9 @@ def sources() -> list[Path]:
 def command(out: Path = OUT) -> list[str]:
     return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-Wall", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            # the device code's LDS atomics are issued by one lane with a wave-uniform address
+            # (queue appends); the atomic optimizer's per-lane scan around them is pure overhead
+            "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
             f"-I{CSRC}", f"-I{INCLUDE}", "-o", str(out)] + [str(s) for s in sources()]
 
 

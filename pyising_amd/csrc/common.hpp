@@ -227,7 +227,8 @@ __device__ __forceinline__ void block_sum2_u64(u64& a, u64& b)
 
 // out[0] = ka * A + ca, out[1] = kb * B + cb with (A, B) the sums of the nblocks partial pairs
 // (one 256-thread workgroup). Nothing needs clearing between calls: every partial is rewritten.
-static __global__ void __launch_bounds__(256) obs_finalize(const unsigned long long* partials,
+// Static (one copy per source file that launches it); unused in the others.
+[[maybe_unused]] static __global__ void __launch_bounds__(256) obs_finalize(const unsigned long long* partials,
                                                           u32 nblocks, long long* out,
                                                           long long ka, long long ca, long long kb,
                                                           long long cb)

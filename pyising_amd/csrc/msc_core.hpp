@@ -271,7 +271,7 @@ __device__ __forceinline__ void msc_two_levels(u32 x, u32 y, u32 a1, u32& pw, u3
 {
     const u32 m = und & msc_teq<KA>(x, a1);
     const u32 u2 = m & msc_teq<KB>(y, a1);
-    if constexpr (KA == 1 && KB == 1) pw ^= und ^ u2;
+    if constexpr (KA == 1 && KB == 1) pw = xor3(pw, und, u2);
     else pw ^= msc_tacc<KA>(und, x, a1) ^ msc_tacc<KB>(m, y, a1);
     und = u2;
 }
@@ -399,12 +399,13 @@ __device__ __forceinline__ MscState msc_unpark(const MscQueueLds<R>& L, u32 slot
     return s;
 }
 
-// Append the lanes of `keep` to queue `q` (one LDS atomic per wave) and park their state.
+// Append the lanes of `keep` (km = its ballot, taken by the caller before any branch so that
+// it stays a scalar mask) to queue `q` (one LDS atomic per wave) and park their state.
 template <int R>
-__device__ __forceinline__ void msc_enqueue(MscQueueLds<R>& L, uint16_t* q, u32* qn, bool keep,
-                                            u32 slot, const MscState& s, bool state_a1)
+__device__ __forceinline__ void msc_enqueue(MscQueueLds<R>& L, uint16_t* q, u32* qn, u64 km,
+                                            bool keep, u32 slot, const MscState& s,
+                                            bool state_a1)
 {
-    const u64 km = __ballot(keep);
     u32 qb = 0;
     if ((threadIdx.x & 63u) == 0u && km) qb = atomicAdd(qn, (u32)__popcll(km));
     qb = __builtin_amdgcn_readlane(qb, 0);
@@ -470,7 +471,10 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
     static_assert(CAP >= 1 && CAP <= 15, "phase A levels");
     const u32 tid = threadIdx.x;
     const u32 lane = tid & 63u;
-    const u32 wave = tid >> 6;
+    // wave index and queue lengths are wave-uniform: held in scalar registers, the phase-B
+    // level lp stays scalar too and each call's case switch is a scalar branch, not a
+    // per-lane branch tree under exec masks
+    const u32 wave = uni(tid >> 6);
     u32 ones = 0;
     if (tid < 18u) L.qn[tid] = 0u;
     __syncthreads();
@@ -509,26 +513,29 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
             msc_pair_switch((u32)(codes >> (4u * (u32)lp)) & 15u, r[lp], s);
         }
         const bool park = (s.ul | s.uh) != 0u;  // invalid lanes have und = 0
+        const u64 km = __ballot(park);
         if (valid && !park) {
             msc_store<NT>(a.own + w, s);
             if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
         }
-        msc_enqueue(L, L.q[0], &L.qn[0], park, slot, s, true);
+        msc_enqueue(L, L.q[0], &L.qn[0], km, park, slot, s, true);
     }
     __syncthreads();
     prefetch();  // e.g. the next tile's loads, which then complete while phase B runs
-    u32 n = L.qn[0];
+    u32 n = uni(L.qn[0]);
     u32 cur = 0;
     u32 lp = CAP;
     while (n > 64u && lp < 16u) {  // block-uniform
         const u32 code = (u32)(codes >> (4u * lp)) & 15u;
         for (u32 i0 = wave * 64u; i0 < n; i0 += 256u) {  // wave-uniform
+            // lanes past the queue's end (i >= n >= 1) repeat its last entry with nothing
+            // undecided, so they neither store nor re-queue
             const u32 i = i0 + lane;
-            MscState s = {};
-            u32 slot = 0;
-            if (i < n) {
-                slot = L.q[cur][i];
-                s = msc_unpark(L, slot);
+            const u32 slot = L.q[cur][i < n ? i : n - 1u];
+            MscState s = msc_unpark(L, slot);
+            if (i >= n) {
+                s.ul = 0u;
+                s.uh = 0u;
             }
             if (STATS) {
                 calls += (s.ul | s.uh) != 0u ? 1u : 0u;
@@ -537,24 +544,25 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
             const PhiloxOut r = msc_philox(a, msc_counter(base + slot), lp);
             msc_pair_switch(code, r, s);
             const bool still = (s.ul | s.uh) != 0u;
+            const u64 km = __ballot(still);
             if (i < n && !still) {
                 msc_store<NT>(a.own + base + slot, s);
                 if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
             }
-            msc_enqueue(L, L.q[cur ^ 1u], &L.qn[lp + 1u - CAP], still, slot, s, false);
+            msc_enqueue(L, L.q[cur ^ 1u], &L.qn[lp + 1u - CAP], km, still, slot, s, false);
         }
         __syncthreads();
-        n = L.qn[lp + 1u - CAP];
+        n = uni(L.qn[lp + 1u - CAP]);
         cur ^= 1u;
         ++lp;
     }
     for (u32 i0 = wave * 64u; i0 < n; i0 += 256u) {  // n <= 64 unless every level is spent
         const u32 i = i0 + lane;
-        MscState s = {};
-        u32 slot = 0;
-        if (i < n) {
-            slot = L.q[cur][i];
-            s = msc_unpark(L, slot);
+        const u32 slot = L.q[cur][i < n ? i : n - 1u];
+        MscState s = msc_unpark(L, slot);
+        if (i >= n) {
+            s.ul = 0u;
+            s.uh = 0u;
         }
         msc_run<true, STATS>(a, codes, base + slot, lp, 16u, s, calls, slots);
         if (i < n) {
