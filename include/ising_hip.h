@@ -123,8 +123,8 @@ int ising_int8_obs(const int8_t* d_c0, const int8_t* d_c1, long long rows, int l
 /* Process default msc kernel: mode 0 queue compaction (k = 10 * cap + R: cap in 3..6 unrolled
  * Philox calls per word, then packed one-call rounds for the undecided words; R in {2,4} x 256
  * words per workgroup), 1 lockstep, 2 eager (all 16 calls; the msc-eager32 ablation of
- * cupy_backend.py:176-180), 3 by size (default, k = 42: lockstep up to 2^19 words per colour,
- * i.e. 8192^2, queue compaction with k above). All modes give identical words. No reference
+ * cupy_backend.py:176-180), 3 by size (default, k = 42: lockstep up to 2^18 words per colour,
+ * e.g. 4096 x 8192, queue compaction with k above). All modes give identical words. No reference
  * counterpart (kernel selection). */
 int ising_msc_config(int mode, int k);
 /* The concrete kernel (mode 0, 1 or 2, and k for mode 0, else 0) that a half sweep of a

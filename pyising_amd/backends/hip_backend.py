@@ -209,7 +209,8 @@ class HipMultiSpin(_HipBase):
     stream = "msc"
     bytes_per_update = 3.0 / 8.0
     #: kernel mode and its parameter (compaction: 10 * cap + R, see ising_msc_config); the
-    #: process default picks the lockstep tile up to 8192^2 and queue compaction above
+    #: process default picks the lockstep tile up to 2^18 words per colour (4096 x 8192) and
+    #: queue compaction above
     mode = MSC_DEFAULT
     tile_k = 42
 

@@ -121,9 +121,9 @@ __global__ __launch_bounds__(256) void msc_obs_kernel(const u64* __restrict__ w0
 // to MSC_LOCKSTEP_MAX_WORDS words per colour, queue compaction with k above. A half sweep of a
 // small lattice is one generation of short-lived workgroups, so its time is the workgroup's
 // latency, which the lockstep tile (one word per lane, no queue phases) keeps lower; measured
-// on MI355X at T_c: lockstep 1.4x faster at 1024^2, 1.3x at 4096^2, even at 8192^2
-// (2^19 words), queue compaction 1.1x faster from 16384^2.
-constexpr u32 MSC_LOCKSTEP_MAX_WORDS = 1u << 19;
+// on MI355X at T_c (us per half sweep, lockstep / queue): 4.33 / 5.75 at 1024^2, 4.96 / 6.27
+// at 4096^2 (2^17 words), 8.36 / 8.24 at 8192^2 (2^19), 22.6 / 19.4 at 16384^2.
+constexpr u32 MSC_LOCKSTEP_MAX_WORDS = 1u << 18;
 static int g_msc_mode = 3;
 static int g_msc_k = 42;
 

@@ -75,8 +75,8 @@ def test_argument_validation_needs_no_gpu():
 
 
 def test_msc_kernel_by_size_needs_no_gpu():
-    """The default msc kernel is the lockstep tile up to 2^19 words per colour (8192^2) and
-    queue compaction above; explicit modes pass through."""
+    """The default msc kernel is the lockstep tile up to 2^18 words per colour (e.g. 4096 x
+    8192) and queue compaction above; explicit modes pass through."""
     import ctypes as C
 
     lib = _lib.load_library()
@@ -87,8 +87,9 @@ def test_msc_kernel_by_size_needs_no_gpu():
         return mode.value, k.value
 
     assert kernel(1024, 8) == (1, 0)
-    assert kernel(8192, 64) == (1, 0)  # 2^19 words
-    assert kernel(8194, 64) == (0, 42)
+    assert kernel(4096, 64) == (1, 0)  # 2^18 words
+    assert kernel(4098, 64) == (0, 42)
+    assert kernel(8192, 64) == (0, 42)  # 2^19 words
     assert kernel(16384, 128) == (0, 42)
     assert kernel(16384, 128, 3, 44) == (0, 44)
     assert kernel(1024, 8, 0, 32) == (0, 32)
