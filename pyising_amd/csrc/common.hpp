@@ -100,6 +100,8 @@ __device__ __forceinline__ u32 uni(u32 x) { return __builtin_amdgcn_readfirstlan
 // caller passes (hi0, lo0) = c0 * M0, which is loop-invariant over levels. The uniform halves
 // of rounds 0-2 stay on the scalar unit and every per-lane XOR with two uniform inputs becomes
 // one VALU XOR: 36 VALU ops per call. Same outputs as philox4x32_10.
+// C3_LANE: c3 may vary across the wave too (it only enters round 0's XOR; same op count).
+template <bool C3_LANE = false>
 __device__ __forceinline__ PhiloxOut philox_lane(u32 hi0, u32 lo0, u32 c1, u32 c2, u32 c3, u32 k0,
                                                  u32 k1)
 {
@@ -107,7 +109,7 @@ __device__ __forceinline__ PhiloxOut philox_lane(u32 hi0, u32 lo0, u32 c1, u32 c
     const u64 q1 = (u64)c2 * PHILOX_M1;
     const u32 u0 = uni((u32)(q1 >> 32) ^ c1 ^ k0);
     const u32 u1 = (u32)q1;
-    const u32 x2 = hi0 ^ uni(c3 ^ k1);
+    const u32 x2 = C3_LANE ? xor3(hi0, c3, uni(k1)) : hi0 ^ uni(c3 ^ k1);
     const u32 x3 = lo0;
     k0 += PHILOX_W0;
     k1 += PHILOX_W1;

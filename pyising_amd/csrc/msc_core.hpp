@@ -234,6 +234,13 @@ __device__ __forceinline__ PhiloxOut msc_philox(const MscArgs& a, const MscCount
     return philox_lane(c.hi0, c.lo0, a.t, a.color, 1u + lp, a.k0, a.k1);
 }
 
+// The same call with a per-lane level pair.
+__device__ __forceinline__ PhiloxOut msc_philox_lanes(const MscArgs& a, const MscCounter& c,
+                                                      u32 lp)
+{
+    return philox_lane<true>(c.hi0, c.lo0, a.t, a.color, 1u + lp, a.k0, a.k1);
+}
+
 // ------------------------------------------------------------------ lazy comparison ----
 // Both levels of one Philox call in closed form. The threshold bit a lane compares against at
 // one level is 0, 1, a1 (thr4 bit 1, thr8 bit 0) or ~a1 (the reverse): kinds 0-3. With x, y
@@ -425,7 +432,8 @@ __device__ __forceinline__ void msc_enqueue(MscQueueLds<R>& L, uint16_t* q, u32*
 //   at the same level (so the level's case code stays wave-uniform); those still undecided
 //   are queued for the next round. A wave slot is wasted only in the last partial wave of a
 //   round, instead of on every lane that finished before the slowest of its 64 words.
-// Tail: once at most 64 survivors remain, one wave runs them lockstep to completion.
+// Tail: once at most 64 survivors remain, wave 0 finishes them, lockstep, or two level pairs
+// per call when at most 32 remain.
 // ROWW: nw % 64 == 0 (every wave's 64 words lie in one row, see msc_classify_row).
 // Returns, when COUNT, this thread's share of the number of +1 spins written.
 // Phase A's loads of one tile, issued before any of its arithmetic (the words of every round
@@ -556,18 +564,63 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
         cur ^= 1u;
         ++lp;
     }
-    for (u32 i0 = wave * 64u; i0 < n; i0 += 256u) {  // n <= 64 unless every level is spent
-        const u32 i = i0 + lane;
-        const u32 slot = L.q[cur][i < n ? i : n - 1u];
-        MscState s = msc_unpark(L, slot);
-        if (i >= n) {
-            s.ul = 0u;
-            s.uh = 0u;
+    if (n <= 32u) {
+        // Tail of at most 32 survivors, run by wave 0. Lane j < 32 owns survivor j and computes
+        // its level pair lp; lane j + 32 computes level pair lp + 1 of the same word (any lane
+        // may evaluate any (word, level) call: the stream is counter-based) and hands its four
+        // words to the owner through ds_bpermute; the owner applies both pairs in order. One
+        // wave-call thus covers two level pairs where lockstep would spend two.
+        if (wave == 0u && n > 0u) {
+            const u32 j = lane & 31u;
+            const u32 slot = L.q[cur][j < n ? j : n - 1u];
+            MscState s = msc_unpark(L, slot);
+            if (lane >= n) {  // helpers (lane >= 32) and owners past the queue's end
+                s.ul = 0u;
+                s.uh = 0u;
+            }
+            const MscCounter ctr = msc_counter(base + slot);
+            const int from = (int)(((lane + 32u) & 63u) << 2);
+            while (lp < 16u && __ballot((s.ul | s.uh) != 0u) != 0) {  // wave-uniform
+                if (STATS) {
+                    calls += (s.ul | s.uh) != 0u ? 1u : 0u;
+                    ++slots;
+                }
+                if (lp + 1u < 16u) {
+                    const PhiloxOut r = msc_philox_lanes(a, ctr, lp + (lane >> 5));
+                    PhiloxOut h;
+                    h.x = (u32)__builtin_amdgcn_ds_bpermute(from, (int)r.x);
+                    h.y = (u32)__builtin_amdgcn_ds_bpermute(from, (int)r.y);
+                    h.z = (u32)__builtin_amdgcn_ds_bpermute(from, (int)r.z);
+                    h.w = (u32)__builtin_amdgcn_ds_bpermute(from, (int)r.w);
+                    msc_pair_switch((u32)(codes >> (4u * lp)) & 15u, r, s);
+                    if (STATS) calls += (s.ul | s.uh) != 0u ? 1u : 0u;
+                    msc_pair_switch((u32)(codes >> (4u * (lp + 1u))) & 15u, h, s);
+                    lp += 2u;
+                } else {
+                    msc_pair_switch((u32)(codes >> (4u * lp)) & 15u, msc_philox(a, ctr, lp), s);
+                    ++lp;
+                }
+            }
+            if (lane < n) {
+                msc_store<NT>(a.own + base + slot, s);
+                if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
+            }
         }
-        msc_run<true, STATS>(a, codes, base + slot, lp, 16u, s, calls, slots);
-        if (i < n) {
-            msc_store<NT>(a.own + base + slot, s);
-            if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
+    } else {
+        // 33..64 survivors (or more once every level is spent): wave 0 runs them lockstep
+        for (u32 i0 = wave * 64u; i0 < n; i0 += 256u) {
+            const u32 i = i0 + lane;
+            const u32 slot = L.q[cur][i < n ? i : n - 1u];
+            MscState s = msc_unpark(L, slot);
+            if (i >= n) {
+                s.ul = 0u;
+                s.uh = 0u;
+            }
+            msc_run<true, STATS>(a, codes, base + slot, lp, 16u, s, calls, slots);
+            if (i < n) {
+                msc_store<NT>(a.own + base + slot, s);
+                if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
+            }
         }
     }
     __syncthreads();  // the LDS of this tile may be reused by the caller's next tile
