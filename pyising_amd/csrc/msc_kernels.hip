@@ -36,6 +36,8 @@
 //    words; streaming stores keep the new spins out of the kernel-boundary L2 write-back.
 #include "msc_core.hpp"
 
+#include <atomic>
+
 // ---------------------------------------------------------------------------- kernels ----
 // stats (optional, diagnostics only): [0] += Philox calls the words needed, [1] += lane-slots
 // the SIMDs executed (64 x wave iterations), [2] += words.
@@ -124,15 +126,17 @@ __global__ __launch_bounds__(256) void msc_obs_kernel(const u64* __restrict__ w0
 // on MI355X at T_c (us per half sweep, lockstep / queue): 4.33 / 5.75 at 1024^2, 4.96 / 6.27
 // at 4096^2 (2^17 words), 8.36 / 8.24 at 8192^2 (2^19), 22.6 / 19.4 at 16384^2.
 constexpr u32 MSC_LOCKSTEP_MAX_WORDS = 1u << 18;
-static int g_msc_mode = 3;
-static int g_msc_k = 42;
+// The process default (mode << 8 | k): one atomic word, so that a launch on one host thread
+// never reads a mode without its k while another thread calls ising_msc_config.
+static std::atomic<int> g_msc_default{(3 << 8) | 42};
 
 // the concrete mode (0, 1 or 2) and k for a lattice of n_words words per colour
 static void msc_resolve(int* mode, int* k, u32 n_words)
 {
     if (*mode < 0) {
-        *mode = g_msc_mode;
-        *k = g_msc_k;
+        const int d = g_msc_default.load(std::memory_order_relaxed);
+        *mode = d >> 8;
+        *k = d & 0xFF;
     }
     if (*mode == 3) *mode = n_words <= MSC_LOCKSTEP_MAX_WORDS ? 1 : 0;
 }
@@ -271,8 +275,11 @@ int ising_msc_config(int mode, int k)
     if (mode < 0) return ising_fail(ISING_EINVAL, "msc_config: mode must be >= 0");
     int rc = msc_mode_ok(mode, k);
     if (rc) return rc;
-    g_msc_mode = mode;
-    if (mode == 0 || mode == 3) g_msc_k = k;
+    int d = g_msc_default.load(std::memory_order_relaxed);
+    int next;
+    do {  // modes 1 and 2 keep the compaction k set before
+        next = (mode << 8) | ((mode == 0 || mode == 3) ? k : (d & 0xFF));
+    } while (!g_msc_default.compare_exchange_weak(d, next, std::memory_order_relaxed));
     return ISING_OK;
 }
 

@@ -29,3 +29,19 @@ def pytest_collection_modifyitems(config, items):
     for item in items:
         if "gpu" in item.keywords:
             item.add_marker(skip)
+
+
+@pytest.fixture(autouse=True, scope="session")
+def _session_tune_cache(tmp_path_factory):
+    """The batch autotuner caches its choices in a file; tests use a fresh one of their own,
+    never the user's cache."""
+    import os
+
+    key = "PYISING_AMD_TUNE_CACHE"
+    old = os.environ.get(key)
+    os.environ[key] = str(tmp_path_factory.mktemp("tune") / "batch_tune.json")
+    yield
+    if old is None:
+        os.environ.pop(key, None)
+    else:
+        os.environ[key] = old

@@ -97,3 +97,29 @@ def test_msc_kernel_by_size_needs_no_gpu():
     assert kernel(16384, 128, 2, 0) == (2, 0)
     assert lib.ising_msc_kernel(1024, 8, 5, 42, C.byref(mode), C.byref(k)) == -1
     assert lib.ising_msc_kernel(1024, 8, 3, 41, C.byref(mode), C.byref(k)) == -1
+
+
+def test_msc_process_default_needs_no_gpu():
+    """ising_msc_config sets the process default that mode -1 resolves to; modes 1 and 2 keep
+    the compaction k set before, which mode 0 / 3 then use again."""
+    import ctypes as C
+
+    lib = _lib.load_library()
+    mode, k = C.c_int(), C.c_int()
+
+    def default(rows, nw):
+        assert lib.ising_msc_kernel(rows, nw, -1, 0, C.byref(mode), C.byref(k)) == 0
+        return mode.value, k.value
+
+    try:
+        assert lib.ising_msc_config(0, 52) == 0
+        assert default(1024, 8) == (0, 52)
+        assert lib.ising_msc_config(1, 0) == 0
+        assert default(16384, 128) == (1, 0)
+        assert lib.ising_msc_config(0, 99) == -1  # rejected: the default is unchanged
+        assert default(16384, 128) == (1, 0)
+        assert lib.ising_msc_config(3, 32) == 0
+        assert default(1024, 8) == (1, 0) and default(16384, 128) == (0, 32)
+    finally:
+        assert lib.ising_msc_config(3, 42) == 0
+    assert default(16384, 128) == (0, 42)
