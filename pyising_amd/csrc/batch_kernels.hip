@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "int8_core.hpp"
@@ -601,7 +602,8 @@ int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, 
     std::vector<u64> seq;
     batch_schedule(descs, seq);
 
-    BatchHandle* h = new BatchHandle();
+    BatchHandle* h = new (std::nothrow) BatchHandle();  // no exception crosses the C ABI
+    if (!h) return ising_fail(ISING_EINVAL, "batch: out of host memory");
     h->msc_mode = (u32)msc_mode;
     h->msc_cap = (u32)msc_cap;
     int rc = batch_alloc(h, (u32)n, tiles, blocks_per_cu);
@@ -645,7 +647,8 @@ int ising_batch_reserve(int max_lattices, long long max_tiles, int blocks_per_cu
         return ising_fail(ISING_EINVAL, "batch_reserve: bad argument");
     if (int rc = batch_check_modes(msc_mode, msc_cap)) return rc;
     *out = nullptr;
-    BatchHandle* h = new BatchHandle();
+    BatchHandle* h = new (std::nothrow) BatchHandle();  // no exception crosses the C ABI
+    if (!h) return ising_fail(ISING_EINVAL, "batch: out of host memory");
     h->msc_mode = (u32)msc_mode;
     h->msc_cap = (u32)msc_cap;
     int rc = batch_alloc(h, (u32)max_lattices, (u32)max_tiles, blocks_per_cu);
