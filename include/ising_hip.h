@@ -179,6 +179,11 @@ typedef struct {
     int vec;             /* int8: sites per lane, 16, 8 or 4 dividing lx/2; 0 = the widest that
                           * still fills one 256-lane tile (else the narrowest). Any width gives
                           * the same spins. */
+    uint32_t max_sweeps; /* sweeps this lattice takes part in, counted from the batch's create
+                          * or assign (0: no limit); past it the lattice's tiles and moment
+                          * sums are skipped, so it holds exactly t0 + max_sweeps sweeps while
+                          * the others go on (continuous batching: no step is cut short to
+                          * the lattice that needs the fewest sweeps) */
 } IsingBatchLattice;
 /* blocks_per_cu <= 0: occupancy maximum; msc_mode 0 = queue-compaction tiles (512 words),
  * 1 = lockstep tiles (256 words); msc_cap = unrolled calls of the queue tiles (must be 4). */

@@ -29,14 +29,15 @@ class IsingBatchLattice(C.Structure):
     _fields_ = [("kind", C.c_int), ("ndim", C.c_int), ("heatbath", C.c_int), ("lz", C.c_int),
                 ("ly", C.c_int), ("lx", C.c_int), ("seed", C.c_ulonglong),
                 ("table", C.c_uint32 * 7), ("c0", C.c_void_p), ("c1", C.c_void_p),
-                ("t0", C.c_uint32), ("vec", C.c_int)]
+                ("t0", C.c_uint32), ("vec", C.c_int), ("max_sweeps", C.c_uint32)]
 
 
 #: msc tile kinds of IsingBatchLattice.kind (1 = the batch's default tile)
 MSC_TILE_KIND = {"default": 1, "lockstep": 2, "queue": 3}
 
 
-def _describe(sim: IsingSimulation, msc_tile: str = "default", vec: int = 0) -> IsingBatchLattice:
+def _describe(sim: IsingSimulation, msc_tile: str = "default", vec: int = 0,
+              max_sweeps: int = 0) -> IsingBatchLattice:
     d = IsingBatchLattice()
     shape = sim.shape
     d.ndim = len(shape)
@@ -58,6 +59,9 @@ def _describe(sim: IsingSimulation, msc_tile: str = "default", vec: int = 0) -> 
     else:
         raise TypeError(f"{type(sim).__name__} is not a hip simulation")
     d.t0 = sim.t & 0xFFFFFFFF
+    if not 0 <= int(max_sweeps) <= 0xFFFFFFFF:
+        raise ValueError("max_sweeps must be in [0, 2^32)")
+    d.max_sweeps = int(max_sweeps)
     return d
 
 
@@ -107,21 +111,26 @@ class BatchScan:
     def __init__(self, sims, blocks_per_cu: int | None = None, msc_mode: int = 0,
                  msc_cap: int = 4, persistent: bool = False, stream: _lib.Stream | None = None,
                  msc_tiles: dict | None = None, int8_vecs: dict | None = None,
-                 moments: bool = True):
+                 moments: bool = True, limits=None):
         """``msc_tiles``: optional {shape: "queue" | "lockstep"} choosing the tile of the msc
         lattices of that shape (others use ``msc_mode``); ``int8_vecs``: optional {shape: 16 |
         8 | 4} sites per lane of the int8 lattices of that shape (others: the widest width that
         fills a tile); e.g. from :func:`pyising_amd.autotune.tune_batch`. ``moments``: False
         skips the per-sweep accumulation of M moments (one launch per sweep fewer; moments()
-        then reports no samples)."""
+        then reports no samples). ``limits``: optional per-lattice number of sweeps it takes
+        part in (0 = no limit); past it a lattice and its moment sums stay as they are."""
         self.sims = list(sims)
         if not self.sims:
             raise ValueError("empty batch")
         self._L = _lib.lib()
         tiles, vecs = msc_tiles or {}, int8_vecs or {}
+        lim = [0] * len(self.sims) if limits is None else [int(x) for x in limits]
+        if len(lim) != len(self.sims):
+            raise ValueError("one sweep limit per lattice")
         arr = (IsingBatchLattice * len(self.sims))(
-            *[_describe(s, tiles.get(tuple(s.shape), "default"), vecs.get(tuple(s.shape), 0))
-              for s in self.sims])
+            *[_describe(s, tiles.get(tuple(s.shape), "default"), vecs.get(tuple(s.shape), 0), n)
+              for s, n in zip(self.sims, lim, strict=True)])
+        self._left = [n if n else None for n in lim]
         h = C.c_void_p()
         _lib.check(self._L.ising_batch_create(arr, len(self.sims), int(blocks_per_cu or 0),
                                               int(msc_mode), int(msc_cap), C.byref(h)),
@@ -155,8 +164,12 @@ class BatchScan:
         return {"grid": g.value, "tiles_per_half_sweep": t.value, "max_blocks_per_cu": m.value}
 
     def _advance(self, n: int) -> None:
-        for s in self.sims:
-            s.t += n
+        for i, s in enumerate(self.sims):
+            left = self._left[i]
+            step = n if left is None else min(n, left)
+            s.t += step
+            if left is not None:
+                self._left[i] = left - step
 
     def sweep(self, n: int = 1) -> None:
         """n sweeps of every lattice in one persistent launch (asynchronous on the batch stream)."""
@@ -215,8 +228,12 @@ class BatchScan:
             pass
 
 
-def _describe_all(sims) -> C.Array:
-    return (IsingBatchLattice * max(1, len(sims)))(*[_describe(s) for s in sims])
+def _describe_all(sims, limits=None) -> C.Array:
+    limits = [0] * len(sims) if limits is None else list(limits)
+    if len(limits) != len(sims):
+        raise ValueError("one sweep limit per lattice")
+    return (IsingBatchLattice * max(1, len(sims)))(
+        *[_describe(s, max_sweeps=n) for s, n in zip(sims, limits, strict=True)])
 
 
 def batch_tiles(sims, msc_mode: int = 0) -> int:
@@ -249,20 +266,31 @@ class ReservedBatch:
                    "batch_set_moments")
         self.max_lattices, self.max_tiles = int(max_lattices), int(max_tiles)
         self.sims: list[IsingSimulation] = []
+        self._left: list[int | None] = []  # sweeps each lattice may still take (None: any)
 
-    def assign(self, sims) -> None:
+    def assign(self, sims, limits=None) -> None:
+        """Replace the lattice list. ``limits``: per lattice, the number of sweeps it takes
+        part in from now on (0 or None: no limit); a lattice past its limit sits the batch's
+        further sweeps out, so one batch sweep count can serve requests of any length."""
         sims = list(sims)
-        _lib.check(self._L.ising_batch_assign(self._h, _describe_all(sims), len(sims),
-                                              self.stream.ptr), "batch_assign")
+        desc = _describe_all(sims, limits)
+        _lib.check(self._L.ising_batch_assign(self._h, desc, len(sims), self.stream.ptr),
+                   "batch_assign")
         self.sims = sims
+        self._left = [int(n) if n else None for n in (limits or [0] * len(sims))]
 
     def sweep(self, n: int = 1) -> None:
         n = int(n)
         if n <= 0 or not self.sims:
             return
         _lib.check(self._L.ising_batch_sweeps(self._h, n, self.stream.ptr), "batch_sweeps")
-        for s in self.sims:
-            s.t += n
+        for i, s in enumerate(self.sims):
+            left = self._left[i]
+            if left is None:
+                s.t += n
+            else:
+                s.t += min(n, left)
+                self._left[i] = left - min(n, left)
 
     def info(self) -> dict:
         g, t, m = C.c_int(), C.c_int(), C.c_int()

@@ -27,6 +27,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -35,6 +36,12 @@
 #include "msc_core.hpp"
 
 namespace cg = cooperative_groups;
+
+// The C ABI struct's layout, which the Python mirror (pyising_amd/batch.py) repeats.
+static_assert(sizeof(IsingBatchLattice) == 96 && offsetof(IsingBatchLattice, t0) == 80 &&
+                  offsetof(IsingBatchLattice, vec) == 84 &&
+                  offsetof(IsingBatchLattice, max_sweeps) == 88,
+              "IsingBatchLattice layout");
 
 // Lattice kinds of the C ABI: int8, msc with the batch's default tile, msc with lockstep
 // 256-word tiles, msc with queue-compaction 512-word tiles (the last two let a tuner choose the
@@ -53,6 +60,7 @@ struct BatchDesc {
     FastDiv div_nw;
     u64 codes;
     u32 k0, k1, t0;
+    u32 limit;  // batch sweeps (since create / assign) this lattice takes part in
     u32 tile_begin, n_tiles;
     long long n_spins;
 };
@@ -121,9 +129,11 @@ __device__ __forceinline__ void batch_fold_moments(BatchAcc& A, const BatchDesc&
     A.samples += 1;
 }
 
-__device__ __forceinline__ void batch_accumulate_lattice(const BatchKernelArgs& ka, u32 lat, u32 par)
+__device__ __forceinline__ void batch_accumulate_lattice(const BatchKernelArgs& ka, u32 lat, u32 par,
+                                                         u32 sw)
 {
     const BatchDesc& d = ka.descs[lat];
+    if (sw >= d.limit) return;  // the lattice sat this sweep out (wave-uniform)
     u64 plus = batch_lattice_plus(ka, d, par, threadIdx.x & 63u, 64u);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) plus += __shfl_xor(plus, off, 64);
@@ -208,6 +218,7 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
             for (u32 tile = blockIdx.x; tile < ka.total_tiles; tile += gridDim.x) {
                 const u32 lat = ka.tile_lat[tile];
                 const BatchDesc& d = ka.descs[lat];
+                if (sw >= d.limit) continue;  // past the lattice's sweep limit (block-uniform)
                 const u32 local = tile - d.tile_begin;
                 const u32 t = d.t0 + sw;
                 u32 plus = 0;
@@ -239,7 +250,8 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
         // M of every lattice, one wave each; this parity's slots are next written two grid
         // barriers later, so the next sweep can start right away.
         const u32 gwave = gtid >> 6, gwaves = gthreads >> 6;
-        for (u32 lat = gwave; lat < ka.n_lat; lat += gwaves) batch_accumulate_lattice(ka, lat, par);
+        for (u32 lat = gwave; lat < ka.n_lat; lat += gwaves)
+            batch_accumulate_lattice(ka, lat, par, sw);
     }
     grid.sync();
     if (gtid == 0) *ka.sweeps_done = done0 + ka.n_sweeps;
@@ -265,6 +277,7 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color,
     if (acc_prev && blockIdx.x >= ka.n_seq) {
         const u32 lat = blockIdx.x - ka.n_seq;
         const BatchDesc& dl = ka.descs[lat];
+        if (sw - 1u >= dl.limit) return;  // the lattice sat the previous sweep out
         u64 plus = batch_lattice_plus(ka, dl, (sw - 1u) & 1u, threadIdx.x, 256u), unused = 0;
         block_sum2_u64(plus, unused);
         if (threadIdx.x == 0) batch_fold_moments(ka.acc[lat], dl, plus);
@@ -275,6 +288,7 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color,
     const u32 par = sw & 1u;
     const u32 tile = (u32)entry, lat = (u32)(entry >> 32);
     const BatchDesc& d = ka.descs[lat];
+    if (sw >= d.limit) return;  // past the lattice's sweep limit: the whole workgroup idles
     const u32 local = tile - d.tile_begin;
     const u32 t = d.t0 + sw;
     u32 plus = 0;
@@ -309,6 +323,7 @@ __global__ __launch_bounds__(256) void batch_accumulate(BatchKernelArgs ka)
 {
     const u32 sw = (u32)ka.sweeps_done[1];
     const BatchDesc& d = ka.descs[blockIdx.x];
+    if (sw >= d.limit) return;
     u64 plus = batch_lattice_plus(ka, d, sw & 1u, threadIdx.x, 256u), unused = 0;
     block_sum2_u64(plus, unused);
     if (threadIdx.x == 0) batch_fold_moments(ka.acc[blockIdx.x], d, plus);
@@ -397,6 +412,7 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
         d.k0 = (u32)(L.seed & 0xFFFFFFFFull);
         d.k1 = (u32)(L.seed >> 32);
         d.t0 = L.t0;
+        d.limit = L.max_sweeps ? L.max_sweeps : 0xFFFFFFFFu;
         d.n_spins = (long long)L.lz * L.ly * L.lx;
         const long long rows = (long long)L.lz * L.ly;
         if (L.kind == BATCH_INT8) {

@@ -13,9 +13,11 @@ seed, a hot start) and gets back the exact final (M, E). Design:
 * Streams. Admissions (device hot start, split into colour arrays) and result reads run on a
   setup stream; batched sweeps run on a work stream. Both are non-blocking streams, and the loop
   calls no hipMalloc/hipFree, so admission work overlaps the sweeps on the device.
-* Continuous batching. Every step sweeps all active lattices ``step_sweeps`` times (fewer if a
+* Continuous batching. Every step runs ``step_sweeps`` batch sweeps (fewer if every active
   request finishes sooner) through the batched kernels of :mod:`pyising_amd.batch`, one launch
-  per half sweep over all lattices. While a step runs, new arrivals are admitted and prepared;
+  per half sweep over all lattices; each lattice carries its own sweep limit, so a request that
+  needs fewer sweeps than the step sits the rest of the step out instead of cutting the step
+  short for all the others. While a step runs, new arrivals are admitted and prepared;
   they join at the next step boundary, when the batch's lattice table is replaced in place
   (``ising_batch_assign``, stream-ordered, no allocation). Finished requests are read and freed
   between steps.
@@ -438,9 +440,12 @@ class Server:
                 continue
             ids = sorted(active)
             if ids != members:
-                self.batch.assign([active[i].sim for i in ids])
+                # each lattice is limited to its own remaining sweeps: it sits out the rest of
+                # a step once done, so a step need not be cut to the shortest request
+                self.batch.assign([active[i].sim for i in ids],
+                                  [active[i].remaining for i in ids])
                 members = ids
-            step = min(self.step_sweeps, min(active[i].remaining for i in ids))
+            step = min(self.step_sweeps, max(active[i].remaining for i in ids))
             self.batch.sweep(step)
             step_ev.record(self.work.ptr)
             rep.steps += 1
@@ -456,7 +461,7 @@ class Server:
                     step_ev.sync()
                     break
             for i in ids:
-                active[i].remaining -= step
+                active[i].remaining -= min(step, active[i].remaining)
         self.setup.sync()
         rep.wall = clock()
         return rep

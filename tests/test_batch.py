@@ -175,12 +175,13 @@ def test_batch_rejects_bad_input(hip):
 def test_struct_layout_matches_header():
     from pyising_amd.batch import IsingBatchLattice
 
-    assert C.sizeof(IsingBatchLattice) == 88
+    assert C.sizeof(IsingBatchLattice) == 96
     assert IsingBatchLattice.seed.offset == 24
     assert IsingBatchLattice.table.offset == 32
     assert IsingBatchLattice.c0.offset == 64
     assert IsingBatchLattice.t0.offset == 80
     assert IsingBatchLattice.vec.offset == 84
+    assert IsingBatchLattice.max_sweeps.offset == 88
 
 
 def test_moments_math():
@@ -198,3 +199,58 @@ def test_moments_math():
     assert ordered.binder == pytest.approx(2 / 3)
     empty = Moments(0, 0, 0, 0, 0, 16)
     assert np.isnan(empty.binder)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("persistent", [False, True])
+def test_sweep_limits_stop_each_lattice_at_its_own_count(hip, persistent):
+    """Per-lattice sweep limits: every lattice ends bit-identical to the oracle after exactly
+    min(limit, sweeps run) sweeps, and its moment sums cover exactly those sweeps."""
+    from pyising_amd.batch import BatchScan
+
+    specs = _mixed(hip)
+    limits = [3, 0, 7, 1, 9, 2, 0, 5, 4, 6]
+    sims = [_make(s) for s in specs]
+    refs = [_make(s) for s in specs]
+    scan = BatchScan(sims, persistent=persistent, limits=limits)
+    scan.sweep(4)
+    scan.sweep(4)
+    scan.synchronize()
+    mom = scan.moments()
+    for sim, ref, lim, mo in zip(sims, refs, limits, mom, strict=True):
+        n = 8 if lim == 0 else min(8, lim)
+        ms = []
+        for _ in range(n):
+            ref.sweep(1)
+            ms.append(int(ref.observables_exact()[0]))
+        assert sim.t == ref.t == n
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+        assert mo.samples == n and mo.sum_m == sum(ms) and mo.sum_m2 == sum(m * m for m in ms)
+    scan.close()
+
+
+@pytest.mark.gpu
+def test_reserved_batch_limits_across_assigns(hip):
+    """A reserved batch re-assigned with fresh limits between steps (the serving pattern)."""
+    from pyising_amd import _lib
+    from pyising_amd.batch import ReservedBatch
+
+    specs = _mixed(hip)
+    sims = [_make(s) for s in specs]
+    refs = [_make(s) for s in specs]
+    st = _lib.Stream()
+    rb = ReservedBatch(len(sims), 4096, st, msc_mode=1)
+    left = [5, 12, 3, 8, 1, 10, 7, 2, 6, 4]
+    while any(left):
+        ids = [i for i, n in enumerate(left) if n]
+        rb.assign([sims[i] for i in ids], [left[i] for i in ids])
+        step = min(4, max(left[i] for i in ids))
+        rb.sweep(step)
+        for i in ids:
+            left[i] -= min(step, left[i])
+    st.sync()
+    for sim, ref, n in zip(sims, refs, [5, 12, 3, 8, 1, 10, 7, 2, 6, 4], strict=True):
+        ref.sweep(n)
+        assert sim.t == ref.t == n
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    rb.close()
