@@ -168,11 +168,13 @@ template <int R, int CAP, int LAY>
 static void launch_queue3(u32 n_tiles, hipStream_t st, const MscArgs& a, u64 codes,
                           unsigned long long* stats)
 {
-    // R = 2: the last min(1024, a quarter of the words / 256) tiles are 256-word tiles
+    // R = 2: the last min(512, a quarter of the words / 256) tiles are 256-word tiles: about
+    // two per CU end the launch (same-box timing at T_c, us per half sweep at 8192^2 / 16384^2:
+    // 384 of them 8.83 / 19.63, 512 8.46 / 19.23, 768 8.47 / 19.42, 1024 8.47 / 19.52)
     u32 n_big = n_tiles, grid = n_tiles;
     if (R == 2) {
         u32 small = a.n_words / 1024u;  // 256-word tiles in a quarter of the words
-        if (small > 1024u) small = 1024u;
+        if (small > 512u) small = 512u;
         small &= ~1u;
         n_big = (a.n_words - small * 256u) / 512u;
         grid = n_big + (a.n_words - n_big * 512u + 255u) / 256u;
