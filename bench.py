@@ -228,8 +228,8 @@ def main() -> int:
     ap.add_argument("--no-workloads", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path "
-              "is design-only in this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); sharding a lattice "
+              "or a batch across GPUs is designed but not implemented", file=sys.stderr)
         return 2
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py is single-process; WORLD_SIZE > 1 is not supported", file=sys.stderr)

@@ -210,3 +210,17 @@ def test_cli_run_and_bench(capsys):
     assert cli.main(["bench", "--variant", "msc", "--L", "1024", "--sweeps", "50",
                      "--repeats", "2"]) == 0
     assert "hip/msc L=1024 dim=2: median" in capsys.readouterr().out
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with status 2 and says why, before any
+    device work."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
