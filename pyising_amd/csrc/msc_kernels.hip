@@ -31,7 +31,7 @@
 //    one row (Lx % 8192 == 0) its row pointers are scalar and the horizontal boundary bit
 //    moves between lanes by a DPP wavefront shift;
 //  * queue compaction (mode 0): `cap` Philox calls per word unrolled (computed before the spin
-//    loads are consumed, hiding their latency), then the ~15% of words still undecided run
+//    loads are consumed, hiding their latency), then the ~19% of words still undecided run
 //    one call per round packed into full waves, so lockstep no longer pays the slowest of 64
 //    words; streaming stores keep the new spins out of the kernel-boundary L2 write-back.
 #include "msc_core.hpp"
