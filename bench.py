@@ -139,7 +139,8 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
             "speedup": round(batch_rate / plain_rate, 2),
             "tuning": {"msc_tiles": {"x".join(map(str, k)): v for k, v in tuning.msc_tiles.items()},
                        "int8_vecs": {"x".join(map(str, k)): v for k, v in tuning.int8_vecs.items()},
-                       "persistent": tuning.persistent, "graph_sweeps": tuning.graph_sweeps},
+                       "persistent": tuning.persistent, "graph_sweeps": tuning.graph_sweeps,
+                       "tiles_per_workgroup": tuning.tiles_per_workgroup},
             "binder_L4096_Tmid": round(mom[-9].binder, 4)}
 
 

@@ -207,6 +207,10 @@ int ising_batch_set_persistent(void* handle, int persistent);
 /* 1 (default): accumulate the moments of M after every sweep (one more launch per sweep in the
  * launch form); 0: skip it, e.g. in a serving loop that reads each lattice's own (M, E). */
 int ising_batch_set_moments(void* handle, int on);
+/* Launch form: tiles each workgroup runs in sequence, k = 1 (default), 2, 4 or 8 (the tiles of
+ * one workgroup come from one XCD's list, so locality is kept). Synchronous: waits for the
+ * device, then rebuilds the launch order. Results do not depend on k. */
+int ising_batch_set_tiles_per_workgroup(void* handle, int k);
 int ising_batch_destroy(void* handle);
 /* Tiles of one half sweep of these lattices (host only, no device call): the tile capacity a
  * reserved batch needs for them. */

@@ -71,6 +71,7 @@ SIGNATURES: dict[str, list] = {
     "ising_batch_info": [_P, _P, _P, _P],
     "ising_batch_set_persistent": [_P, _I],
     "ising_batch_set_moments": [_P, _I],
+    "ising_batch_set_tiles_per_workgroup": [_P, _I],
     "ising_batch_destroy": [_P],
     "ising_batch_tiles": [_P, _I, _I, _P],
     "ising_batch_reserve": [_I, _LL, _I, _I, _I, _P],

@@ -21,6 +21,7 @@ from dataclasses import asdict, dataclass, field
 from pathlib import Path
 
 GRAPH_CHOICES = (0, 10, 50)
+TILES_PER_WG_CHOICES = (1, 2, 4)  # launch form: tiles (lattices or parts of one) per workgroup
 
 
 @dataclass
@@ -29,6 +30,7 @@ class BatchTuning:
     int8_vecs: dict = field(default_factory=dict)  # shape tuple -> 16 | 8 | 4 (absent: default)
     persistent: bool = False
     graph_sweeps: int = 50
+    tiles_per_workgroup: int = 1
     flips_per_ns: float = 0.0
     trials: list = field(default_factory=list)
 
@@ -45,6 +47,7 @@ class BatchTuning:
                 for k, v in d.get("int8_vecs", {}).items()}
         return cls(msc_tiles=tiles, int8_vecs=vecs, persistent=bool(d.get("persistent", False)),
                    graph_sweeps=int(d.get("graph_sweeps", 50)),
+                   tiles_per_workgroup=int(d.get("tiles_per_workgroup", 1)),
                    flips_per_ns=float(d.get("flips_per_ns", 0.0)), trials=d.get("trials", []))
 
 
@@ -99,7 +102,7 @@ def _rate(scratch, t: BatchTuning, sweeps: int) -> float:
 
     graph_sweeps = t.graph_sweeps
     scan = BatchScan(scratch, msc_tiles=t.msc_tiles, int8_vecs=t.int8_vecs,
-                     persistent=t.persistent)
+                     persistent=t.persistent, tiles_per_workgroup=t.tiles_per_workgroup)
     try:
         n_spins = int(scan.n_spins.sum())
         e0, e1 = _lib.Event(), _lib.Event()
@@ -129,7 +132,8 @@ def _vec_choices(shape) -> list[int]:
 
 
 def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
-    """Pick the msc tile and int8 width per shape, the launch form and the graph length."""
+    """Pick the msc tile and int8 width per shape, the launch form, the tiles each workgroup
+    runs in sequence (launch form) and the graph length."""
     key = signature(sims)
     cache = _load_cache() if use_cache else {}
     if key in cache:
@@ -142,7 +146,8 @@ def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
 
     def copy(**kw) -> BatchTuning:
         base = {"msc_tiles": dict(best.msc_tiles), "int8_vecs": dict(best.int8_vecs),
-                "persistent": best.persistent, "graph_sweeps": best.graph_sweeps}
+                "persistent": best.persistent, "graph_sweeps": best.graph_sweeps,
+                "tiles_per_workgroup": best.tiles_per_workgroup}
         base.update(kw)
         return BatchTuning(**base)
 
@@ -151,6 +156,7 @@ def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
         trials.append({"tiles": {"x".join(map(str, k)): v for k, v in t.msc_tiles.items()},
                        "int8_vecs": {"x".join(map(str, k)): v for k, v in t.int8_vecs.items()},
                        "persistent": t.persistent, "graph_sweeps": t.graph_sweeps,
+                       "tiles_per_workgroup": t.tiles_per_workgroup,
                        "flips_per_ns": round(r, 1)})
         return r
 
@@ -171,6 +177,10 @@ def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
             vecs[sh] = v
             consider(copy(int8_vecs=vecs))
     consider(copy(persistent=not best.persistent))
+    if not best.persistent:
+        for k in TILES_PER_WG_CHOICES:
+            if k != best.tiles_per_workgroup:
+                consider(copy(tiles_per_workgroup=k))
     for g in GRAPH_CHOICES:
         if g != best.graph_sweeps:
             consider(copy(graph_sweeps=g))

@@ -111,14 +111,16 @@ class BatchScan:
     def __init__(self, sims, blocks_per_cu: int | None = None, msc_mode: int = 0,
                  msc_cap: int = 4, persistent: bool = False, stream: _lib.Stream | None = None,
                  msc_tiles: dict | None = None, int8_vecs: dict | None = None,
-                 moments: bool = True, limits=None):
+                 moments: bool = True, limits=None, tiles_per_workgroup: int = 1):
         """``msc_tiles``: optional {shape: "queue" | "lockstep"} choosing the tile of the msc
         lattices of that shape (others use ``msc_mode``); ``int8_vecs``: optional {shape: 16 |
         8 | 4} sites per lane of the int8 lattices of that shape (others: the widest width that
         fills a tile); e.g. from :func:`pyising_amd.autotune.tune_batch`. ``moments``: False
         skips the per-sweep accumulation of M moments (one launch per sweep fewer; moments()
         then reports no samples). ``limits``: optional per-lattice number of sweeps it takes
-        part in (0 = no limit); past it a lattice and its moment sums stay as they are."""
+        part in (0 = no limit); past it a lattice and its moment sums stay as they are.
+        ``tiles_per_workgroup``: launch form, tiles each workgroup runs in sequence (1, 2, 4,
+        8; the autotuner's lattices-per-workgroup schedule). No setting changes a result."""
         self.sims = list(sims)
         if not self.sims:
             raise ValueError("empty batch")
@@ -140,6 +142,11 @@ class BatchScan:
                    "batch_set_persistent")
         _lib.check(self._L.ising_batch_set_moments(self._h, int(bool(moments))),
                    "batch_set_moments")
+        if int(tiles_per_workgroup) != 1:
+            _lib.check(self._L.ising_batch_set_tiles_per_workgroup(self._h,
+                                                                   int(tiles_per_workgroup)),
+                       "batch_set_tiles_per_workgroup")
+        self.tiles_per_workgroup = int(tiles_per_workgroup)
         self.persistent = bool(persistent)
         self.n_spins = np.array([s.n_spins for s in self.sims], dtype=np.int64)
         self.stream = stream if stream is not None else _lib.Stream()
@@ -152,7 +159,8 @@ class BatchScan:
         from .autotune import tune_batch
 
         t = tune_batch(sims, sweeps=sweeps, use_cache=use_cache)
-        scan = cls(sims, persistent=t.persistent, msc_tiles=t.msc_tiles, int8_vecs=t.int8_vecs)
+        scan = cls(sims, persistent=t.persistent, msc_tiles=t.msc_tiles, int8_vecs=t.int8_vecs,
+                   tiles_per_workgroup=t.tiles_per_workgroup)
         scan.tuning = t
         if t.graph_sweeps and not scan.capture(t.graph_sweeps):
             scan.graph_sweeps = 0
@@ -162,6 +170,14 @@ class BatchScan:
         g, t, m = C.c_int(), C.c_int(), C.c_int()
         _lib.check(self._L.ising_batch_info(self._h, C.byref(g), C.byref(t), C.byref(m)))
         return {"grid": g.value, "tiles_per_half_sweep": t.value, "max_blocks_per_cu": m.value}
+
+    def set_tiles_per_workgroup(self, k: int) -> None:
+        """Tiles each workgroup of the launch form runs in sequence (1, 2, 4 or 8). Waits for
+        the device; a captured graph is dropped (capture again to replay)."""
+        _lib.check(self._L.ising_batch_set_tiles_per_workgroup(self._h, int(k)),
+                   "batch_set_tiles_per_workgroup")
+        self.tiles_per_workgroup = int(k)
+        self.graph_sweeps = 0
 
     def _advance(self, n: int) -> None:
         for i, s in enumerate(self.sims):

@@ -86,7 +86,8 @@ struct BatchKernelArgs {
     unsigned long long* sweeps_done;
     u32 n_lat, total_tiles, n_sweeps, msc_mode, msc_cap;
     u32 moments;  // launch form: 0 = nobody reads the per-tile counts, skip storing them
-    u32 n_seq;    // launch form: tile workgroups per half sweep (the launch order's length)
+    u32 n_seq;    // launch form: tile workgroups per half sweep
+    u32 per_wg;   // launch form: launch-order entries (tiles) each of them runs in sequence
 };
 
 // Per-wave sum of v stored to the wave's own slot, dst[wave] (four slots per tile): no atomics
@@ -265,6 +266,9 @@ __global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
 // acc_prev (colour 0 only): the n_lat workgroups after the n_seq tile workgroups accumulate the
 // moments of the previous sweep, whose count slots (the other parity) this sweep does not write;
 // this saves the accumulation launch of every sweep but a call's last.
+// K = ka.per_wg tiles per workgroup, a template parameter: the one-tile form keeps the
+// register count of a single tile body (a runtime loop costs 17 VGPRs and with them occupancy).
+template <u32 K>
 __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color, u32 acc_prev)
 {
     __shared__ MscQueueLds<2> lds;
@@ -283,38 +287,41 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color,
         if (threadIdx.x == 0) batch_fold_moments(ka.acc[lat], dl, plus);
         return;
     }
-    const u64 entry = ka.tile_seq[blockIdx.x];
-    if (entry == BATCH_NO_TILE) return;  // padding of a shorter XCD list (whole workgroup)
     const u32 par = sw & 1u;
-    const u32 tile = (u32)entry, lat = (u32)(entry >> 32);
-    const BatchDesc& d = ka.descs[lat];
-    if (sw >= d.limit) return;  // past the lattice's sweep limit: the whole workgroup idles
-    const u32 local = tile - d.tile_begin;
-    const u32 t = d.t0 + sw;
-    u32 plus = 0;
-    if (d.kind == BATCH_INT8) {
-        if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1, false>(d, thr_lds, local, t, color)
-                                     : batch_int8_v<2, 0, false>(d, thr_lds, local, t, color);
-        else plus = d.hb ? batch_int8_v<3, 1, false>(d, thr_lds, local, t, color)
-                         : batch_int8_v<3, 0, false>(d, thr_lds, local, t, color);
-    } else {
-        MscArgs a;
-        a.own = (u64*)(color ? d.c1 : d.c0);
-        a.other = (const u64*)(color ? d.c0 : d.c1);
-        a.rows = d.rows;
-        a.nw = d.nw;
-        a.n_words = d.n_words;
-        a.div_nw = d.div_nw;
-        a.t = t;
-        a.color = color;
-        a.k0 = d.k0;
-        a.k1 = d.k1;
-        plus = d.msc_queue ? batch_msc_tile(a, d.codes, local * 512u, d.layout, lds)
-                           : batch_msc_lockstep(a, d.codes, local, d.layout);
+    for (u32 j = 0; j < K; ++j) {  // block-uniform
+        const u64 entry = ka.tile_seq[(size_t)blockIdx.x * K + j];
+        if (entry == BATCH_NO_TILE) continue;  // padding of a shorter XCD list (whole workgroup)
+        const u32 tile = (u32)entry, lat = (u32)(entry >> 32);
+        const BatchDesc& d = ka.descs[lat];
+        if (sw >= d.limit) continue;  // past the lattice's sweep limit: the whole workgroup idles
+        if (j) __syncthreads();  // an earlier tile of this workgroup may still read thr_lds
+        const u32 local = tile - d.tile_begin;
+        const u32 t = d.t0 + sw;
+        u32 plus = 0;
+        if (d.kind == BATCH_INT8) {
+            if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1, false>(d, thr_lds, local, t, color)
+                                         : batch_int8_v<2, 0, false>(d, thr_lds, local, t, color);
+            else plus = d.hb ? batch_int8_v<3, 1, false>(d, thr_lds, local, t, color)
+                             : batch_int8_v<3, 0, false>(d, thr_lds, local, t, color);
+        } else {
+            MscArgs a;
+            a.own = (u64*)(color ? d.c1 : d.c0);
+            a.other = (const u64*)(color ? d.c0 : d.c1);
+            a.rows = d.rows;
+            a.nw = d.nw;
+            a.n_words = d.n_words;
+            a.div_nw = d.div_nw;
+            a.t = t;
+            a.color = color;
+            a.k0 = d.k0;
+            a.k1 = d.k1;
+            plus = d.msc_queue ? batch_msc_tile(a, d.codes, local * 512u, d.layout, lds)
+                               : batch_msc_lockstep(a, d.codes, local, d.layout);
+        }
+        if (ka.moments)
+            wave_store_sum(ka.plus + ((size_t)par * 2 + color) * BATCH_WAVES * ka.total_tiles +
+                               (size_t)BATCH_WAVES * tile, plus);
     }
-    if (ka.moments)
-        wave_store_sum(ka.plus + ((size_t)par * 2 + color) * BATCH_WAVES * ka.total_tiles +
-                           (size_t)BATCH_WAVES * tile, plus);
 }
 
 // After both colours of a sweep (when moments are on): one workgroup per lattice, spread over
@@ -339,8 +346,10 @@ __global__ void batch_zero_u64(unsigned long long* p, u32 n)
 // ising_hip.h.
 
 struct BatchHandle {
-    u32 n_lat = 0, total_tiles = 0, n_seq = 0;
+    u32 n_lat = 0, total_tiles = 0, n_seq = 0;  // n_seq: tile workgroups of the launch form
+    u32 per_wg = 1;  // launch form: tiles per workgroup (the launch order has n_seq * per_wg)
     u32 cap_lat = 0, cap_tiles = 0, cap_seq = 0;  // device table capacity (reserved: > n_lat)
+    std::vector<BatchDesc> host_descs;  // the current lattices (to rebuild the launch order)
     BatchDesc* d_descs = nullptr;
     u32* d_tile_lat = nullptr;
     u64* d_tile_seq = nullptr;
@@ -481,7 +490,9 @@ static u32 batch_tile_cost(const BatchDesc& d)
     return d.msc_queue ? 3u : 2u;
 }
 
-static void batch_schedule(const std::vector<BatchDesc>& descs, std::vector<u64>& seq)
+// Launch order of the launch form with k tiles per workgroup: workgroup w (on XCD w % 8) runs
+// entries [w k, w k + k) of the order, which are consecutive tiles of XCD (w % 8)'s list.
+static void batch_schedule(const std::vector<BatchDesc>& descs, u32 k, std::vector<u64>& seq)
 {
     constexpr u32 X = 8;
     std::vector<u32> order(descs.size());
@@ -506,16 +517,19 @@ static void batch_schedule(const std::vector<BatchDesc>& descs, std::vector<u64>
     }
     size_t longest = 0;
     for (u32 x = 0; x < X; ++x) longest = lists[x].size() > longest ? lists[x].size() : longest;
-    seq.assign(longest * X, BATCH_NO_TILE);
+    const size_t rows = (longest + k - 1) / k;  // workgroups per XCD
+    seq.assign(rows * X * k, BATCH_NO_TILE);
     for (u32 x = 0; x < X; ++x)
-        for (size_t k = 0; k < lists[x].size(); ++k) seq[k * X + x] = lists[x][k];
+        for (size_t i = 0; i < lists[x].size(); ++i)
+            seq[((i / k) * X + x) * k + i % k] = lists[x][i];
 }
 
 // Capacity of the launch order for cap_lat lattices and cap_tiles tiles: each lattice can
-// lengthen the longest list by at most one tile beyond its even share.
+// lengthen the longest list by at most one tile beyond its even share, and k tiles per
+// workgroup pad each list by at most 7 more.
 static u32 batch_seq_capacity(u32 cap_lat, u32 cap_tiles)
 {
-    return cap_tiles + 8u * (cap_lat + 1u);
+    return cap_tiles + 8u * (cap_lat + 8u);
 }
 
 // Allocates the device tables for cap_lat lattices and cap_tiles tiles and sets the grid
@@ -561,6 +575,17 @@ static int batch_zero_stats(BatchHandle* h, hipStream_t st)
     return ISING_OK;
 }
 
+static void batch_launch_half(u32 per_wg, dim3 grid, hipStream_t st, const BatchKernelArgs& ka,
+                              u32 color, u32 acc_prev)
+{
+    switch (per_wg) {
+    case 2: hipLaunchKernelGGL(batch_half<2>, grid, dim3(256), 0, st, ka, color, acc_prev); break;
+    case 4: hipLaunchKernelGGL(batch_half<4>, grid, dim3(256), 0, st, ka, color, acc_prev); break;
+    case 8: hipLaunchKernelGGL(batch_half<8>, grid, dim3(256), 0, st, ka, color, acc_prev); break;
+    default: hipLaunchKernelGGL(batch_half<1>, grid, dim3(256), 0, st, ka, color, acc_prev); break;
+    }
+}
+
 static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
 {
     if (h->total_tiles == 0) return ISING_OK;  // an empty reserved batch
@@ -571,6 +596,7 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
     ka.plus = h->d_plus;
     ka.moments = (u32)h->moments;
     ka.n_seq = h->n_seq;
+    ka.per_wg = h->per_wg;
     ka.acc = h->d_acc;
     ka.sweeps_done = h->d_done;
     ka.n_lat = h->n_lat;
@@ -586,9 +612,9 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
     }
     for (u32 s = 0; s < n_sweeps; ++s) {
         const u32 acc_prev = (h->moments && s > 0) ? 1u : 0u;
-        hipLaunchKernelGGL(batch_half, dim3(h->n_seq + (acc_prev ? h->n_lat : 0u)), dim3(256), 0,
-                           st, ka, 0u, acc_prev);
-        hipLaunchKernelGGL(batch_half, dim3(h->n_seq), dim3(256), 0, st, ka, 1u, 0u);
+        batch_launch_half(h->per_wg, dim3(h->n_seq + (acc_prev ? h->n_lat : 0u)), st, ka, 0u,
+                          acc_prev);
+        batch_launch_half(h->per_wg, dim3(h->n_seq), st, ka, 1u, 0u);
     }
     // the last sweep's moments (earlier ones were folded into the next colour-0 launch)
     if (h->moments) hipLaunchKernelGGL(batch_accumulate, dim3(h->n_lat), dim3(256), 0, st, ka);
@@ -616,7 +642,7 @@ int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, 
     if (int rc = batch_build(lats, n, (u32)msc_mode, descs, tile_lat)) return rc;
     const u32 tiles = (u32)tile_lat.size();
     std::vector<u64> seq;
-    batch_schedule(descs, seq);
+    batch_schedule(descs, 1u, seq);
 
     BatchHandle* h = new (std::nothrow) BatchHandle();  // no exception crosses the C ABI
     if (!h) return ising_fail(ISING_EINVAL, "batch: out of host memory");
@@ -628,6 +654,7 @@ int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, 
         return rc;
     }
     batch_set_sizes(h, (u32)n, tiles, (u32)seq.size());
+    h->host_descs = descs;
     hipError_t e = hipMemcpy(h->d_descs, descs.data(), sizeof(BatchDesc) * (size_t)n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_tile_lat, tile_lat.data(), sizeof(u32) * tile_lat.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_tile_seq, seq.data(), sizeof(u64) * seq.size(), hipMemcpyHostToDevice);
@@ -702,7 +729,7 @@ int ising_batch_assign(void* handle, const IsingBatchLattice* lats, int n, void*
     if (int rc = batch_build(lats, n, h->msc_mode, descs, tile_lat)) return rc;
     if (tile_lat.size() > h->cap_tiles) return ising_fail(ISING_ERANGE, "batch_assign: more tiles than reserved");
     std::vector<u64> seq;
-    batch_schedule(descs, seq);
+    batch_schedule(descs, h->per_wg, seq);
     if (seq.size() > h->cap_seq) return ising_fail(ISING_ERANGE, "batch_assign: launch order exceeds its capacity");
     hipStream_t st = as_stream(stream);
     const int s = h->next_slot;
@@ -725,7 +752,8 @@ int ising_batch_assign(void* handle, const IsingBatchLattice* lats, int n, void*
     }
     ISING_HIP(hipEventRecord(h->pinned_done[s], st));
     batch_drop_graph(h);
-    batch_set_sizes(h, (u32)n, (u32)tile_lat.size(), (u32)seq.size());
+    batch_set_sizes(h, (u32)n, (u32)tile_lat.size(), (u32)(seq.size() / h->per_wg));
+    h->host_descs = std::move(descs);
     return batch_zero_stats(h, st);
 }
 
@@ -822,6 +850,30 @@ int ising_batch_set_moments(void* handle, int on)
     if (!h || (on != 0 && on != 1)) return ising_fail(ISING_EINVAL, "batch_set_moments: bad argument");
     batch_drop_graph(h);
     h->moments = on;
+    return ISING_OK;
+}
+
+// Launch form: each tile workgroup runs k (1, 2, 4 or 8) consecutive tiles of its XCD's list in
+// sequence (fewer, longer-lived workgroups; one tile each by default). Rebuilds and uploads the
+// launch order after a device synchronisation (no launch may be in flight on any stream);
+// results do not depend on k. Destroys a captured graph.
+int ising_batch_set_tiles_per_workgroup(void* handle, int k)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || (k != 1 && k != 2 && k != 4 && k != 8))
+        return ising_fail(ISING_EINVAL, "batch_set_tiles_per_workgroup: k must be 1, 2, 4 or 8");
+    if ((u32)k == h->per_wg) return ISING_OK;
+    batch_drop_graph(h);
+    std::vector<u64> seq;
+    if (h->n_lat) {
+        batch_schedule(h->host_descs, (u32)k, seq);
+        if (seq.size() > h->cap_seq)
+            return ising_fail(ISING_ERANGE, "batch_set_tiles_per_workgroup: launch order exceeds its capacity");
+        ISING_HIP(hipDeviceSynchronize());  // launches queued earlier read the old order
+        ISING_HIP(hipMemcpy(h->d_tile_seq, seq.data(), sizeof(u64) * seq.size(), hipMemcpyHostToDevice));
+    }
+    h->per_wg = (u32)k;
+    if (h->n_lat) batch_set_sizes(h, h->n_lat, h->total_tiles, (u32)(seq.size() / (u32)k));
     return ISING_OK;
 }
 

@@ -27,7 +27,7 @@ def test_tuning_json_round_trip(tmp_path, monkeypatch):
 
     t = BatchTuning(msc_tiles={(1024, 1024): "lockstep", (128, 256): "queue"},
                     int8_vecs={(64, 64): 8, (8, 8, 16): 4}, persistent=True,
-                    graph_sweeps=10, flips_per_ns=123.4)
+                    graph_sweeps=10, tiles_per_workgroup=4, flips_per_ns=123.4)
     assert BatchTuning.from_json(t.to_json()) == t
     monkeypatch.setenv("PYISING_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
     autotune._save_cache({"k": t.to_json()})

@@ -254,3 +254,40 @@ def test_reserved_batch_limits_across_assigns(hip):
         assert sim.t == ref.t == n
         np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
     rb.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [2, 4, 8])
+def test_tiles_per_workgroup_change_nothing(hip, k):
+    """k tiles per workgroup (the lattices-per-workgroup schedule): lattices and moment sums
+    equal the one-tile form and the oracle, also with k changed between calls and under graph
+    replay; bad k is rejected."""
+    from pyising_amd import _lib
+    from pyising_amd.batch import BatchScan
+
+    specs = _mixed(hip)
+    sims = [_make(s) for s in specs]
+    base = [_make(s) for s in specs]
+    refs = [_make(s) for s in specs]
+    scan = BatchScan(sims, tiles_per_workgroup=k)
+    ref_scan = BatchScan(base)
+    scan.sweep(3)
+    scan.set_tiles_per_workgroup(1)
+    scan.sweep(2)
+    scan.set_tiles_per_workgroup(k)
+    assert scan.capture(2)
+    scan.replay(2)
+    ref_scan.sweep(9)
+    scan.synchronize()
+    ref_scan.synchronize()
+    for sim, b, ref in zip(sims, base, refs, strict=True):
+        ref.sweep(9)
+        assert sim.t == b.t == ref.t == 9
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+        np.testing.assert_array_equal(b.get_lattice(), ref.get_lattice())
+    for m, mb in zip(scan.moments(), ref_scan.moments(), strict=True):
+        assert (m.samples, m.sum_m, m.sum_m2, m.sum_m4) == (mb.samples, mb.sum_m, mb.sum_m2, mb.sum_m4)
+    lib = _lib.load_library()
+    assert lib.ising_batch_set_tiles_per_workgroup(C.c_void_p(scan._h), 3) == -1
+    scan.close()
+    ref_scan.close()
