@@ -159,10 +159,11 @@ int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, in
 /* No reference counterpart: the reference sweeps one lattice per launch sequence
  * (src/pyising/backends/base.py:85-90). A batch is a list of lattices (int8 2D/3D either
  * algorithm, or msc 2D), each with its own shape, thresholds, seed and sweep counter, whose
- * device colour arrays stay owned by the caller. One persistent cooperative kernel sweeps
- * them all (grid barrier between the two half sweeps); every lattice's result is
- * bit-identical to sweeping it alone. After every sweep M = sum of spins is accumulated per
- * lattice: sum M, sum |M|, sum M^2 (int64) and sum M^4 (128-bit) for Binder cumulants
+ * device colour arrays stay owned by the caller. Every half sweep of all of them is one launch
+ * (default), or a whole call is one persistent cooperative launch with a grid barrier between
+ * half sweeps (ising_batch_set_persistent); every lattice's result is bit-identical to sweeping
+ * it alone. After every sweep M = sum of spins is accumulated per lattice: sum M, sum |M|,
+ * sum M^2 (int64) and sum M^4 (128-bit) for Binder cumulants
  * (src/pyising/observables.py:39-42). */
 typedef struct {
     int kind;            /* 0 = int8 compressed checkerboard; msc packed: 1 = the batch's
@@ -189,7 +190,7 @@ typedef struct {
  * 1 = lockstep tiles (256 words); msc_cap = unrolled calls of the queue tiles (must be 4). */
 int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, int msc_mode,
                        int msc_cap, void** handle);
-/* n_sweeps sweeps of every lattice in one cooperative launch (asynchronous). */
+/* n_sweeps sweeps of every lattice (asynchronous; launch form or persistent form). */
 int ising_batch_sweeps(void* handle, long long n_sweeps, void* stream);
 /* Capture one launch of sweeps_per_replay sweeps into a hipGraph on `stream` (a created
  * stream); ISING_EHIP if the runtime cannot capture it. Replays continue the sweep counter. */
