@@ -1,21 +1,21 @@
-// Batched temperature scans: one persistent kernel sweeps many lattices of uneven size,
-// dimension, variant and temperature in a single launch (no reference counterpart; the
-// reference runs one lattice per launch sequence, src/pyising/backends/base.py:85-90).
+// Batched temperature scans: one launch per half sweep (or one persistent launch per call)
+// sweeps many lattices of uneven size, dimension, variant and temperature (no reference
+// counterpart; the reference runs one lattice per launch sequence,
+// src/pyising/backends/base.py:85-90).
 //
 // Work decomposition. Every lattice is cut into tiles of one workgroup's work for one half
 // sweep: int8 = 256 threads x V sites, msc = 512 words (queue compaction) or 256 words
-// (lockstep). A tile map lists (tile -> lattice) for all lattices; the grid strides over
-// it. Colour c + 1 of a lattice may only start when every tile of colour c of that lattice is
-// done: the kernel is launched cooperatively (all workgroups co-resident, checked by the
-// runtime) and a grid barrier separates the two half sweeps of every sweep, which orders all
-// lattices at once. Each tile is bit-identical to the single-lattice kernels (same device
+// (lockstep). Colour c + 1 of a lattice may only start when every tile of colour c of that
+// lattice is done. Each tile is bit-identical to the single-lattice kernels (same device
 // functions, same counters: t = t0 + sweep), so a batched lattice equals the same lattice swept
-// alone.
+// alone; a lattice with a sweep limit sits the batch's later sweeps out.
 //
-// Two forms give identical results: a persistent cooperative launch per call (grid barrier
-// between half sweeps), and (default) one launch per half sweep over all lattices plus one
-// accumulation launch per sweep. On MI355X the kernel boundary is the cheaper barrier: a grid
-// barrier is a contended cross-XCD atomic on one address per workgroup.
+// Two forms give identical results. Default: one launch per half sweep over all lattices (the
+// kernel boundary orders the colours), workgroups taking 1, 2, 4 or 8 tiles of a launch order
+// that balances every lattice over the eight XCDs, long tiles first. Persistent: one
+// cooperative launch per call whose grid strides over a tile map, with a grid barrier between
+// half sweeps. On MI355X the kernel boundary is the cheaper barrier: a grid barrier is a
+// contended cross-XCD atomic on one address per workgroup.
 //
 // Binder accumulation. While writing a colour, each tile stores its number of +1 spins in its
 // own slot (slots double-buffered by sweep parity; no atomics, so no contention on a
