@@ -19,7 +19,9 @@
 //
 // Binder accumulation. While writing a colour, each tile stores its number of +1 spins in its
 // own slot (slots double-buffered by sweep parity; no atomics, so no contention on a
-// per-lattice counter). After a sweep, one wave per lattice sums its slots, forms
+// per-lattice counter). After a sweep, one workgroup (launch form: in the next sweep's colour-0
+// launch or, after a call's last sweep, its own launch) or one wave (persistent form) per
+// lattice sums its slots, forms
 // M = 2 * plus - N and accumulates sum M, sum |M|, sum M^2 (int64: M^2 <= 2^48 for L <= 4096,
 // so 10^4 sweeps stay below 2^62) and sum M^4 as a 128-bit (hi, lo) pair. The number of sweeps
 // done lives in device memory, so a captured graph can be replayed and every replay continues
