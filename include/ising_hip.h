@@ -163,7 +163,7 @@ int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, in
  * (default), or a whole call is one persistent cooperative launch with a grid barrier between
  * half sweeps (ising_batch_set_persistent); every lattice's result is bit-identical to sweeping
  * it alone. After every sweep M = sum of spins is accumulated per lattice: sum M, sum |M|,
- * sum M^2 (int64) and sum M^4 (128-bit) for Binder cumulants
+ * sum M^2 and sum M^4 (128-bit) for Binder cumulants
  * (src/pyising/observables.py:39-42). */
 typedef struct {
     int kind;            /* 0 = int8 compressed checkerboard; msc packed: 1 = the batch's
@@ -196,8 +196,8 @@ int ising_batch_sweeps(void* handle, long long n_sweeps, void* stream);
  * stream); ISING_EHIP if the runtime cannot capture it. Replays continue the sweep counter. */
 int ising_batch_graph(void* handle, long long sweeps_per_replay, void* stream);
 int ising_batch_replay(void* handle, int replays, void* stream);
-/* Synchronous read of n x 6 int64 accumulators (sum M, sum |M|, sum M^2, sum M^4 lo, hi,
- * samples) and the number of sweeps done. */
+/* Synchronous read of n x 7 int64 accumulators (sum M, sum |M|, sum M^2 lo, hi, sum M^4 lo,
+ * hi, samples; the 128-bit sums as unsigned (lo, hi) words) and the number of sweeps done. */
 int ising_batch_stats(void* handle, long long* out, long long* sweeps_done, void* stream);
 int ising_batch_reset_stats(void* handle, void* stream);
 int ising_batch_info(void* handle, int* grid, int* total_tiles, int* max_blocks_per_cu);

@@ -215,16 +215,17 @@ class BatchScan:
         self.stream.sync()
 
     def moments(self) -> list[Moments]:
-        out = np.zeros((len(self.sims), 6), dtype=np.int64)
+        out = np.zeros((len(self.sims), 7), dtype=np.int64)
         done = C.c_longlong()
         _lib.check(self._L.ising_batch_stats(self._h, out.ctypes.data, C.byref(done),
                                              self.stream.ptr), "batch_stats")
         res = []
         u = out.view(np.uint64)
         for i, row in enumerate(out):
-            m4 = int(u[i, 3]) | (int(u[i, 4]) << 64)  # 128-bit sum M^4 as (lo, hi)
-            res.append(Moments(samples=int(row[5]), sum_m=int(row[0]), sum_abs_m=int(u[i, 1]),
-                               sum_m2=int(u[i, 2]), sum_m4=m4, n_spins=int(self.n_spins[i])))
+            m2 = int(u[i, 2]) | (int(u[i, 3]) << 64)  # 128-bit sums as (lo, hi)
+            m4 = int(u[i, 4]) | (int(u[i, 5]) << 64)
+            res.append(Moments(samples=int(row[6]), sum_m=int(row[0]), sum_abs_m=int(u[i, 1]),
+                               sum_m2=m2, sum_m4=m4, n_spins=int(self.n_spins[i])))
         self.sweeps_done = done.value
         return res
 

@@ -22,8 +22,8 @@
 // per-lattice counter). After a sweep, one workgroup (launch form: in the next sweep's colour-0
 // launch or, after a call's last sweep, its own launch) or one wave (persistent form) per
 // lattice sums its slots, forms
-// M = 2 * plus - N and accumulates sum M, sum |M|, sum M^2 (int64: M^2 <= 2^48 for L <= 4096,
-// so 10^4 sweeps stay below 2^62) and sum M^4 as a 128-bit (hi, lo) pair. The number of sweeps
+// M = 2 * plus - N and accumulates sum M, sum |M| (64-bit: |M| <= N) and sum M^2, sum M^4 as
+// 128-bit (lo, hi) pairs. The number of sweeps
 // done lives in device memory, so a captured graph can be replayed and every replay continues
 // the counter stream.
 #include <hip/hip_cooperative_groups.h>
@@ -69,7 +69,9 @@ struct BatchDesc {
 
 struct BatchAcc {
     long long sum_m;
-    unsigned long long sum_abs_m, sum_m2, sum_m4_lo, sum_m4_hi, samples;
+    // sum M^2 and sum M^4 as 128-bit (lo, hi) pairs: M^2 <= N^2 reaches 2^56 at 16384^2, so a
+    // 64-bit sum could wrap after a few hundred ordered sweeps
+    unsigned long long sum_abs_m, sum_m2_lo, sum_m2_hi, sum_m4_lo, sum_m4_hi, samples;
 };
 
 constexpr u64 BATCH_NO_TILE = ~0ull;
@@ -104,7 +106,7 @@ __device__ __forceinline__ void wave_store_sum(u32* dst, u32 v)
 }
 
 // One wave: M of lattice `lat` from its tiles' counts of parity `par`, folded into the
-// accumulators (sum M, sum |M|, sum M^2, 128-bit sum M^4, samples).
+// accumulators (sum M, sum |M|, 128-bit sums M^2 and M^4, samples).
 // The +1 spins of lattice `lat` in sweep parity `par`: this thread's share of its count
 // slots, threads `first`, `first + step`, ...
 __device__ __forceinline__ u64 batch_lattice_plus(const BatchKernelArgs& ka, const BatchDesc& d,
@@ -125,7 +127,9 @@ __device__ __forceinline__ void batch_fold_moments(BatchAcc& A, const BatchDesc&
     const u64 lo = m2 * m2, hi = __umul64hi(m2, m2);
     A.sum_m += m;
     A.sum_abs_m += (u64)(m < 0 ? -m : m);
-    A.sum_m2 += m2;
+    const u64 n2 = A.sum_m2_lo + m2;
+    A.sum_m2_hi += n2 < m2 ? 1u : 0u;
+    A.sum_m2_lo = n2;
     const u64 nlo = A.sum_m4_lo + lo;
     A.sum_m4_hi += hi + (nlo < lo ? 1u : 0u);
     A.sum_m4_lo = nlo;

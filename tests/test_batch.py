@@ -291,3 +291,27 @@ def test_tiles_per_workgroup_change_nothing(hip, k):
     assert lib.ising_batch_set_tiles_per_workgroup(C.c_void_p(scan._h), 3) == -1
     scan.close()
     ref_scan.close()
+
+
+@pytest.mark.gpu
+def test_moment_sums_carry_past_64_bits(hip):
+    """An ordered 16384^2 lattice has M^2 ~ 2^56, so 300 sweeps take sum M^2 past 2^64: the
+    128-bit sums must equal sum M^2 and sum M^4 of the exact per-sweep M of the same lattice
+    swept alone."""
+    from pyising_amd.batch import BatchScan
+
+    shape, beta, n = (16384, 16384), 1.0, 300
+    sim = hip.HipMultiSpin(shape, beta, seed=5, init="cold")
+    alone = hip.HipMultiSpin(shape, beta, seed=5, init="cold")
+    scan = BatchScan([sim])
+    scan.sweep(n)
+    mo = scan.moments()[0]
+    ms = []
+    for _ in range(n):
+        alone.sweep(1)
+        ms.append(int(alone.observables_exact()[0]))
+    assert mo.samples == n
+    assert mo.sum_m2 == sum(m * m for m in ms) and mo.sum_m2 >= 2**64
+    assert mo.sum_m4 == sum(m**4 for m in ms)
+    assert mo.sum_abs_m == sum(abs(m) for m in ms) and mo.sum_m == sum(ms)
+    scan.close()
