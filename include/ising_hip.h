@@ -163,7 +163,8 @@ int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, in
  * (default), or a whole call is one persistent cooperative launch with a grid barrier between
  * half sweeps (ising_batch_set_persistent); every lattice's result is bit-identical to sweeping
  * it alone. After every sweep M = sum of spins is accumulated per lattice: sum M, sum |M|,
- * sum M^2 and sum M^4 (128-bit) for Binder cumulants
+ * sum M^2 and sum M^4 (128-bit) for Binder cumulants; a batched lattice holds at most 2^28
+ * spins (ISING_ERANGE beyond), so the sums stay exact for 65536 sweeps
  * (src/pyising/observables.py:39-42). */
 typedef struct {
     int kind;            /* 0 = int8 compressed checkerboard; msc packed: 1 = the batch's

@@ -429,6 +429,9 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
         d.t0 = L.t0;
         d.limit = L.max_sweeps ? L.max_sweeps : 0xFFFFFFFFu;
         d.n_spins = (long long)L.lz * L.ly * L.lx;
+        // exact moments: M^4 <= N^4 = 2^112 at N = 2^28, so the 128-bit sum holds 65536 sweeps
+        if (d.n_spins > (1ll << 28))
+            return ising_fail(ISING_ERANGE, "batch: at most 2^28 spins per lattice (16384^2), for exact 128-bit moment sums");
         const long long rows = (long long)L.lz * L.ly;
         if (L.kind == BATCH_INT8) {
             if (L.ndim != 2 && L.ndim != 3) return ising_fail(ISING_EINVAL, "batch: int8 ndim 2 or 3");

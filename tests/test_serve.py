@@ -299,3 +299,23 @@ def test_server_rejects_without_stopping():
 def test_gpu_mix_is_valid():
     for r in _mix():
         r.validate()
+
+
+def test_batched_lattices_are_bounded_for_exact_moments():
+    """A batched lattice holds at most 2^28 spins (16384^2): the 128-bit M^4 sum then stays exact
+    for 65536 sweeps. Host-only entry point, no GPU needed."""
+    import ctypes as C
+
+    from pyising_amd import _lib
+    from pyising_amd.batch import IsingBatchLattice
+
+    lib = _lib.load_library()
+    tiles = C.c_longlong()
+    for (lx, ok) in ((16384, True), (16512, False)):
+        d = IsingBatchLattice()
+        d.kind, d.ndim, d.lz, d.ly, d.lx = 1, 2, 1, 16384, lx
+        d.c0 = d.c1 = 256
+        rc = lib.ising_batch_tiles(C.byref(d), 1, 0, C.byref(tiles))
+        assert (rc == 0) == ok, (lx, rc)
+        if not ok:
+            assert rc == -3 and b"2^28" in lib.ising_last_error()
