@@ -246,7 +246,7 @@ class BatchScan:
 
 
 def _describe_all(sims, limits=None) -> C.Array:
-    limits = [0] * len(sims) if limits is None else list(limits)
+    limits = [0] * len(sims) if limits is None else [0 if n is None else int(n) for n in limits]
     if len(limits) != len(sims):
         raise ValueError("one sweep limit per lattice")
     return (IsingBatchLattice * max(1, len(sims)))(
