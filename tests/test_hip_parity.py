@@ -304,3 +304,45 @@ def test_config_msc_L16384_tc_few_sweeps(hip):
     w0, w1 = sim.get_words()
     assert np.array_equal(w0, ref.w[0]) and np.array_equal(w1, ref.w[1])
     assert sim.observables_exact() == ref.observables_exact()
+
+
+def _random_cases(n=36, seed=2024):
+    """Seeded draws over the shapes the layouts allow: int8 2D/3D (every dimension even, lx a
+    multiple of 8) with both algorithms, msc 2D (lx a multiple of 128) in every kernel mode."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(n):
+        kind = ("int8_2d", "int8_3d", "msc")[i % 3]
+        beta = float(rng.uniform(0.05, 0.8))
+        seed_i = int(rng.integers(0, 2**63))
+        if kind == "int8_2d":
+            shape = (2 * int(rng.integers(1, 40)), 8 * int(rng.integers(1, 40)))
+            alg = ("metropolis", "heatbath")[int(rng.integers(0, 2))]
+            cases.append((kind, shape, beta, seed_i, alg, None))
+        elif kind == "int8_3d":
+            shape = (2 * int(rng.integers(1, 6)), 2 * int(rng.integers(1, 8)),
+                     8 * int(rng.integers(1, 5)))
+            alg = ("metropolis", "heatbath")[int(rng.integers(0, 2))]
+            cases.append((kind, shape, beta * 0.5, seed_i, alg, None))
+        else:
+            shape = (2 * int(rng.integers(1, 80)), 128 * int(rng.integers(1, 70)))
+            mode = [(0, 42), (0, 32), (1, 0), (2, 0), (3, 42)][int(rng.integers(0, 5))]
+            cases.append((kind, shape, beta, seed_i, "metropolis", mode))
+    return cases
+
+
+@pytest.mark.parametrize("case", _random_cases(), ids=lambda c: f"{c[0]}-{'x'.join(map(str, c[1]))}")
+def test_random_shapes_match_oracle(hip, case):
+    kind, shape, beta, seed, alg, mode = case
+    lat = O.random_lattice(shape, seed % 1000)
+    if kind == "msc":
+        ref = O.MscOracle(shape, beta, seed=seed, init=lat)
+        cls = type("M", (hip.HipMultiSpin,), {"mode": mode[0], "tile_k": mode[1] or 42})
+        sim = cls(shape, beta, seed=seed, init=lat)
+    else:
+        ref = O.Int8Oracle(shape, beta, seed=seed, algorithm=alg, init=lat)
+        sim = hip.HipInt8(shape, beta, seed=seed, algorithm=alg, init=lat)
+    ref.sweep(3)
+    sim.sweep(3)
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    assert sim.observables_exact() == ref.observables_exact()
