@@ -315,3 +315,49 @@ def test_moment_sums_carry_past_64_bits(hip):
     assert mo.sum_m4 == sum(m**4 for m in ms)
     assert mo.sum_abs_m == sum(abs(m) for m in ms) and mo.sum_m == sum(ms)
     scan.close()
+
+
+def _random_mix(rng, hip):
+    specs = []
+    for i in range(int(rng.integers(5, 10))):
+        kind = int(rng.integers(0, 3))
+        beta = float(rng.uniform(0.1, 0.7))
+        if kind == 0:
+            shape = (2 * int(rng.integers(2, 40)), 8 * int(rng.integers(1, 40)))
+            cls, alg = hip.HipInt8, ("metropolis", "heatbath")[int(rng.integers(0, 2))]
+        elif kind == 1:
+            shape = (2 * int(rng.integers(1, 5)), 2 * int(rng.integers(1, 8)),
+                     8 * int(rng.integers(1, 4)))
+            cls, alg, beta = hip.HipInt8, ("metropolis", "heatbath")[int(rng.integers(0, 2))], beta / 2
+        else:
+            shape = (2 * int(rng.integers(1, 60)), 128 * int(rng.integers(1, 40)))
+            cls, alg = hip.HipMultiSpin, "metropolis"
+        specs.append((cls, shape, beta, alg, int(rng.integers(0, 2**40)),
+                      O.random_lattice(shape, int(rng.integers(0, 1000)))))
+    return specs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("draw", range(6))
+def test_random_batches_match_oracle(hip, draw):
+    """Seeded random mixes: shapes, variants, sweep limits, tiles per workgroup, launch form and
+    msc tile kind drawn at random; every lattice must equal the oracle after its own sweeps."""
+    from pyising_amd.batch import BatchScan
+
+    rng = np.random.default_rng(1000 + draw)
+    specs = _random_mix(rng, hip)
+    sims = [_make(s) for s in specs]
+    refs = [_make(s) for s in specs]
+    limits = [int(x) for x in rng.integers(0, 9, len(specs))]
+    persistent = bool(draw % 3 == 2)
+    scan = BatchScan(sims, msc_mode=int(rng.integers(0, 2)), persistent=persistent,
+                     limits=limits, tiles_per_workgroup=int(rng.choice([1, 2, 4, 8])))
+    scan.sweep(3)
+    scan.sweep(4)
+    scan.synchronize()
+    for sim, ref, lim in zip(sims, refs, limits, strict=True):
+        n = 7 if lim == 0 else min(7, lim)
+        ref.sweep(n)
+        assert sim.t == ref.t == n
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    scan.close()
