@@ -171,6 +171,8 @@ def end_states() -> list[dict]:
                     engine="c"))
     out.append(case("msc_L16384_beta_c_hot_3", "msc", (16384, 16384), BETA_C_2D, 1, 3,
                     engine="c"))
+    out.append(case("msc_L32768_beta_c_hot_2", "msc", (32768, 32768), BETA_C_2D, 3, 2,
+                    engine="c"))
     return out
 
 
