@@ -173,6 +173,9 @@ def end_states() -> list[dict]:
                     engine="c"))
     out.append(case("msc_L32768_beta_c_hot_2", "msc", (32768, 32768), BETA_C_2D, 3, 2,
                     engine="c"))
+    for alg in ("metropolis", "heatbath"):
+        out.append(case(f"int8_2d_L16384_beta_c_{alg}_hot_2", "int8", (16384, 16384), BETA_C_2D,
+                        11, 2, alg, engine="c"))
     return out
 
 
