@@ -176,6 +176,14 @@ def end_states() -> list[dict]:
     for alg in ("metropolis", "heatbath"):
         out.append(case(f"int8_2d_L16384_beta_c_{alg}_hot_2", "int8", (16384, 16384), BETA_C_2D,
                         11, 2, alg, engine="c"))
+    # queue-compaction msc in its per-lane (88-word rows) and row-segment (32-word rows)
+    # layouts at sizes above the lockstep threshold, and a large 3D int8 lattice
+    out.append(case("msc_L11264_beta_c_hot_3", "msc", (11264, 11264), BETA_C_2D, 13, 3,
+                    engine="c"))
+    out.append(case("msc_16384x4096_beta_c_hot_3", "msc", (16384, 4096), BETA_C_2D, 17, 3,
+                    engine="c"))
+    out.append(case("int8_3d_L512_Kc_metropolis_hot_2", "int8", (512, 512, 512), BETA_C_3D, 19,
+                    2, engine="c"))
     return out
 
 
