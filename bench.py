@@ -41,35 +41,46 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(seconds: float = 10.0) -> dict:
-    """The oracle's NumPy msc sweep on the same lattice (L=16384, T_c, seed 1, hot start),
-    one host core, timed over whole sweeps until ``seconds`` have passed."""
+def _time_like_reference(step, min_repeat_s: float, repeats: int = 5):
+    """The reference's time_one (bench/run.py:121-142): one warm-up call, then ``repeats``
+    timed repeats of whole calls, each repeat at least ``min_repeat_s`` long; returns
+    (calls per repeat, median seconds per call, total seconds timed)."""
+    step()  # warm-up (allocations, page faults)
+    t0 = time.perf_counter()
+    step()
+    one = time.perf_counter() - t0
+    per_repeat = max(1, int(np.ceil(min_repeat_s / max(one, 1e-9))))
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for _ in range(per_repeat):
+            step()
+        times.append((time.perf_counter() - t0) / per_repeat)
+    return per_repeat, float(np.median(times)), float(sum(times) * per_repeat)
+
+
+def cpu_baseline(min_repeat_s: float = 0.5) -> dict:
+    """The oracle's NumPy msc sweep on the same lattice (L=16384, T_c, seed 1, hot start) on
+    one host core, timed like the reference's bench (warm-up sweep, then the median of 5
+    repeats of whole sweeps, each repeat >= ``min_repeat_s``; one sweep takes ~2.5 s here)."""
     from oracle import ising_oracle as O
     from pyising_amd.lattice import BETA_C_2D
 
     sim = O.MscOracle((L_HEADLINE, L_HEADLINE), BETA_C_2D, seed=1)
-    sim.half_sweep(0)  # warm-up (allocations, page faults)
-    sweeps = 0
-    t0 = time.perf_counter()
-    while True:
-        sim.sweep(1)
-        sweeps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    rate = sweeps * L_HEADLINE**2 / el / 1e9
+    per, sec, total = _time_like_reference(lambda: sim.sweep(1), min_repeat_s)
+    rate = L_HEADLINE**2 / sec / 1e9
     out = {"value": round(rate, 4), "unit": "flips/ns", "cores": 1, "kind": "port",
-           "sample": f"oracle NumPy msc sweep, L={L_HEADLINE} T_c seed 1 hot start, "
-                     f"{sweeps} sweeps in {el:.1f} s on one core of {cpu_model()} "
-                     f"(os.cpu_count()={os.cpu_count()})"}
+           "sample": f"oracle NumPy msc sweep, L={L_HEADLINE} T_c seed 1 hot start: warm-up "
+                     f"sweep, then median of 5 repeats of {per} sweep(s) ({total:.1f} s timed) "
+                     f"on one core of {cpu_model()} (os.cpu_count()={os.cpu_count()})"}
     try:
-        out["openmp_c"] = cpu_baseline_openmp(min(seconds, 5.0))
+        out["openmp_c"] = cpu_baseline_openmp(min_repeat_s)
     except (OSError, RuntimeError) as exc:  # the C restatement was not built
         out["openmp_c"] = {"error": str(exc)[:200]}
     return out
 
 
-def cpu_baseline_openmp(seconds: float = 10.0, threads: int = 16) -> dict:
+def cpu_baseline_openmp(min_repeat_s: float = 0.5, threads: int = 16) -> dict:
     """The oracle's C restatement of the same msc sweep (same lattice and stream) with OpenMP
     over rows on ``threads`` host threads (the box's CPU share), timed like the NumPy one."""
     from oracle import c_oracle as CO
@@ -78,20 +89,12 @@ def cpu_baseline_openmp(seconds: float = 10.0, threads: int = 16) -> dict:
     threads = max(1, min(threads, os.cpu_count() or 1))
     CO.set_threads(threads)
     sim = CO.CMsc((L_HEADLINE, L_HEADLINE), BETA_C_2D, seed=1)
-    sim.sweep(1)  # warm-up
-    sweeps = 0
-    t0 = time.perf_counter()
-    while True:
-        sim.sweep(1)
-        sweeps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    rate = sweeps * L_HEADLINE**2 / el / 1e9
+    per, sec, total = _time_like_reference(lambda: sim.sweep(1), min_repeat_s)
+    rate = L_HEADLINE**2 / sec / 1e9
     return {"value": round(rate, 4), "unit": "flips/ns", "cores": threads, "kind": "port",
             "sample": f"oracle C msc sweep (OpenMP over rows), L={L_HEADLINE} T_c seed 1 hot "
-                      f"start, {sweeps} sweeps in {el:.1f} s on {threads} threads of "
-                      f"{cpu_model()}"}
+                      f"start: warm-up sweep, then median of 5 repeats of {per} sweeps "
+                      f"({total:.1f} s timed) on {threads} threads of {cpu_model()}"}
 
 
 def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
@@ -134,6 +137,8 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
         s.sweep(plain_sweeps)
     _lib.device_sync()
     plain_rate = plain_sweeps * n_spins / (time.perf_counter() - t0) / 1e9
+    for h in list(sims) + [e0, e1]:
+        h.close()
     return {"lattices": len(sims), "spins": int(n_spins), "graph": bool(graph),
             "batched_flips_per_ns": round(batch_rate, 1), "plain_flips_per_ns": round(plain_rate, 1),
             "speedup": round(batch_rate / plain_rate, 2),
@@ -171,11 +176,13 @@ def workloads() -> dict:
 
     sim = HipInt8((1024, 1024), 0.4406868, seed=1, stream=stream.ptr)
     rate, dt, ser = timed(sim, 100)
+    sim.close()
     out["int8_2d_L1024_100_sweeps"] = {"flips_per_ns": round(rate, 1), "ms": round(dt * 1e3, 3),
                                        "M_E_after": list(ser[-1])}
     for T in (1.5, 2.269, 3.0):
         sim = HipInt8((8192, 8192), 1.0 / T, seed=1, stream=stream.ptr)
         rate, dt, ser = timed(sim, 200)
+        sim.close()
         out[f"int8_2d_L8192_T{T}"] = {"flips_per_ns": round(rate, 1), "M_E_after": list(ser[-1])}
     for alg in ("metropolis", "heatbath"):
         sim = HipInt8((256, 256, 256), BETA_C_3D, seed=7, algorithm=alg, stream=stream.ptr)
@@ -184,7 +191,8 @@ def workloads() -> dict:
             "flips_per_ns": round(rate, 1), "ms": round(dt * 1e3, 1),
             "mean_abs_m": round(float(np.mean([abs(m) for m, _ in ser[len(ser) // 2:]])) / 256**3, 5),
             "mean_e": round(float(np.mean([e for _, e in ser[len(ser) // 2:]])) / 256**3, 5)}
-        del sim
+        sim.close()
+    stream.close()
     return out
 
 
@@ -208,13 +216,35 @@ def serving(n_requests: int = 300, rate: float = 200.0, seed: int = 0) -> dict:
     return out
 
 
-def traffic_from_profile() -> int | None:
-    """HBM bytes per msc half-sweep launch from the committed rocprofv3 PMC summary."""
+def settle_clocks(sim, e0, e1, min_ms: float, chunk: int = 250) -> tuple[int, float]:
+    """Sweep ``sim`` in chunks until at least ``min_ms`` of GPU time has passed (events on the
+    simulation's stream); returns (sweeps run, GPU milliseconds)."""
+    sweeps, ms = 0, 0.0
+    while ms < min_ms:
+        e0.record(sim.stream_ptr)
+        sim.sweep(chunk)
+        e1.record(sim.stream_ptr)
+        e1.sync()
+        ms += e0.elapsed_ms(e1)
+        sweeps += chunk
+    return sweeps, ms
+
+
+def traffic_from_profile() -> tuple[int | None, str]:
+    """HBM bytes per msc half-sweep launch from the committed rocprofv3 PMC summary, only if
+    that summary was taken on the kernel sources built now (their digest is recorded in it)."""
+    from pyising_amd.probes import kernel_source_digest
+
     try:
         d = json.loads(PMC_FILE.read_text())
-        return int(d["hbm_bytes_per_launch"])
+        recorded, value = d.get("kernel_sources_sha16"), int(d["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
-        return None
+        return None, f"no PMC summary at {PMC_FILE.name}"
+    now = kernel_source_digest()
+    if recorded != now:
+        return None, (f"{PMC_FILE.name} was taken on kernel sources {recorded}, "
+                      f"built sources are {now}: not used")
+    return value, f"{PMC_FILE.name} (FETCH_SIZE + WRITE_SIZE, kernel sources {now})"
 
 
 def main() -> int:
@@ -222,6 +252,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--settle-ms", type=float, default=300.0,
+                    help="GPU milliseconds of sweeps before --warmup, to settle the clocks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ceilings", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -253,10 +285,13 @@ def main() -> int:
     # the simulation's own non-blocking stream (the `stream=` keyword); the legacy null stream
     # measures 2-4 % lower for the same launches
     stream = _lib.Stream()
+    e0, e1 = _lib.Event(), _lib.Event()
     sim = HipMultiSpin(shape, BETA_C_2D, seed=1, init="hot", stream=stream.ptr)
+    # clock settle (outside --warmup and the timed region): the GPU's clocks need tens of
+    # milliseconds of sustained load to reach their steady level after the idle set-up
+    settle_sweeps, settle_ms = settle_clocks(sim, e0, e1, args.settle_ms)
     sim.sweep(args.warmup)
     sim.synchronize()
-    e0, e1 = _lib.Event(), _lib.Event()
     _lib.device_sync()
     t0 = time.perf_counter()
     e0.record(sim.stream_ptr)
@@ -272,6 +307,7 @@ def main() -> int:
     achieved = algo_bytes / kernel_s / 1e9
     m, e = sim.observables_exact()
     stats = sim.philox_stats(2)
+    traffic, traffic_note = traffic_from_profile()
 
     result = {
         "metric": "spin flips/ns (2D msc Metropolis, L=16384, T=T_c)",
@@ -291,15 +327,20 @@ def main() -> int:
                    "L": L_HEADLINE, "beta": BETA_C_2D, "seed": 1, "kernel": sim.kernel()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile(),
+                     "traffic": traffic, "traffic_source": traffic_note,
                      "algorithmic_bytes_per_launch": int(algo_bytes),
                      "avg_launch_us": round(kernel_s * 1e6, 3)},
+        "clock_settle": {"sweeps": settle_sweeps, "gpu_ms": round(settle_ms, 1),
+                         "note": "sweeps run before --warmup, outside the timed region"},
         "device": {"arch": name.value.decode(), "cu": n_cu.value,
                    "hbm_GiB": round(total.value / 2**30, 1)},
         "final_M": m, "final_E": e,
         "philox_calls_per_word": round(stats["calls_per_word"], 4),
         "philox_slots_per_word": round(stats["slots_per_word"], 4),
     }
+    sim.close()
+    for h in (e0, e1, stream):
+        h.close()
     del sim
     if not args.no_ceilings:
         from pyising_amd.probes import copy_bandwidth, philox_rate

@@ -93,28 +93,53 @@ static inline int neighbour_sum(const int8_t* other, int64_t r, int64_t j, int64
     return h;
 }
 
+/* One row r = z * ly + y of colour `color` (row parity (y + z) % 2, src/pyising/lattice.py:
+ * 43-46): h = other[j] + other[j +- 1] (+1 when (y + z + color) is odd, else -1, wrapping in
+ * the row) + the same column in rows y +- 1 (and planes z +- 1 in 3D), periodic. Colour site
+ * n = r * lh + j takes word n % 4 of Philox((n / 4, t, colour, 0), key); lh % 4 == 0, so a
+ * quad never straddles two rows. */
+static void int8_row(int8_t* own, const int8_t* other, int64_t r, int64_t lh, int64_t ly,
+                     int64_t lz, int ndim, int heatbath, const uint32_t* table, uint32_t t,
+                     uint32_t color, uint32_t k0, uint32_t k1)
+{
+    const int64_t z = r / ly, y = r - z * ly;
+    const int p = (int)((y + z + color) & 1);
+    const int8_t* row = other + r * lh;
+    const int8_t* up = other + (z * ly + (y + 1 == ly ? 0 : y + 1)) * lh;
+    const int8_t* dn = other + (z * ly + (y == 0 ? ly - 1 : y - 1)) * lh;
+    const int8_t* fw = row;
+    const int8_t* bk = row;
+    if (ndim == 3) {
+        fw = other + ((z + 1 == lz ? 0 : z + 1) * ly + y) * lh;
+        bk = other + ((z == 0 ? lz - 1 : z - 1) * ly + y) * lh;
+    }
+    int8_t* o = own + r * lh;
+    for (int64_t j0 = 0; j0 < lh; j0 += 4) {
+        uint32_t c[4] = {(uint32_t)((r * lh + j0) / 4), t, color, 0u};
+        philox10(c, k0, k1);
+        for (int i = 0; i < 4; ++i) {
+            const int64_t j = j0 + i;
+            const int64_t jn = p ? (j + 1 == lh ? 0 : j + 1) : (j == 0 ? lh - 1 : j - 1);
+            int h = row[j] + row[jn] + up[j] + dn[j];
+            if (ndim == 3) h += fw[j] + bk[j];
+            const int s = o[j];
+            if (heatbath) {
+                o[j] = (c[i] < table[(h + 2 * ndim) / 2]) ? 1 : -1;
+            } else {
+                const int x = s * h;
+                o[j] = (x <= 0 || c[i] < table[(x + 2 * ndim) / 2]) ? (int8_t)-s : (int8_t)s;
+            }
+        }
+    }
+}
+
 void oracle_int8_half(int8_t* own, const int8_t* other, int64_t rows, int64_t lh, int64_t ly,
                       int64_t lz, int ndim, int heatbath, const uint32_t* table, uint32_t t,
                       uint32_t color, uint32_t k0, uint32_t k1)
 {
-    const int64_t n_quads = rows * lh / 4;
 #pragma omp parallel for schedule(static)
-    for (int64_t q = 0; q < n_quads; ++q) {
-        uint32_t c[4] = {(uint32_t)q, t, color, 0u};
-        philox10(c, k0, k1);
-        for (int i = 0; i < 4; ++i) {
-            const int64_t n = 4 * q + i;
-            const int64_t r = n / lh, j = n - r * lh;
-            const int h = neighbour_sum(other, r, j, lh, ly, lz, ndim, color);
-            const int s = own[n];
-            if (heatbath) {
-                own[n] = (c[i] < table[(h + 2 * ndim) / 2]) ? 1 : -1;
-            } else {
-                const int x = s * h;
-                own[n] = (x <= 0 || c[i] < table[(x + 2 * ndim) / 2]) ? (int8_t)-s : (int8_t)s;
-            }
-        }
-    }
+    for (int64_t r = 0; r < rows; ++r)
+        int8_row(own, other, r, lh, ly, lz, ndim, heatbath, table, t, color, k0, k1);
 }
 
 void oracle_int8_sweeps(int8_t* c0, int8_t* c1, int64_t rows, int64_t lh, int64_t ly, int64_t lz,

@@ -8,7 +8,9 @@ is visible, :func:`lib` raises.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import sys
 import threading
 from pathlib import Path
 
@@ -85,6 +87,26 @@ class HipError(RuntimeError):
 
 _lib: C.CDLL | None = None
 _lock = threading.Lock()
+_finalizing = False
+
+
+def _mark_finalizing() -> None:
+    global _finalizing
+    _finalizing = True
+
+
+# Runs when the interpreter starts to shut down, before module objects are collected: from
+# then on garbage-collected handles release nothing. A release at that point could run after
+# the HIP runtime (or a profiler's tool library) has begun its own teardown, and a native fault
+# there cannot be caught. Programs release what they hold with the explicit close()/free()
+# calls; the process's exit returns the rest to the driver.
+atexit.register(_mark_finalizing)
+
+
+def releasing_allowed() -> bool:
+    """False once interpreter shutdown has begun: ``__del__`` paths must then call nothing in
+    the library."""
+    return not (_finalizing or sys.is_finalizing())
 
 
 def load_library() -> C.CDLL:
@@ -149,9 +171,11 @@ class DeviceBuffer:
             self.ptr = None
 
     def __del__(self):
+        if not releasing_allowed():
+            return
         try:
             self.free()
-        except Exception:  # noqa: BLE001 - interpreter teardown
+        except Exception:  # noqa: BLE001 - a failing release in a finalizer is dropped
             pass
 
     def upload(self, host, stream=None) -> None:
@@ -210,11 +234,17 @@ class Event:
     def sync(self) -> None:
         check(self._lib.ising_event_sync(self.ptr), "event_sync")
 
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            self._lib.ising_event_destroy(self.ptr)
+            self.ptr = None
+
     def __del__(self):
+        if not releasing_allowed():
+            return
         try:
-            if self.ptr:
-                self._lib.ising_event_destroy(self.ptr)
-        except Exception:  # noqa: BLE001
+            self.close()
+        except Exception:  # noqa: BLE001 - a failing release in a finalizer is dropped
             pass
 
 
@@ -234,11 +264,19 @@ class Stream:
         """Later work on this stream waits for the last record of ``event``."""
         check(self._lib.ising_stream_wait_event(self.ptr, event.ptr), "stream_wait_event")
 
+    def close(self) -> None:
+        """Wait for the stream's work, then destroy it."""
+        if getattr(self, "ptr", None):
+            self._lib.ising_stream_sync(self.ptr)
+            self._lib.ising_stream_destroy(self.ptr)
+            self.ptr = None
+
     def __del__(self):
+        if not releasing_allowed():
+            return
         try:
-            if self.ptr:
-                self._lib.ising_stream_destroy(self.ptr)
-        except Exception:  # noqa: BLE001
+            self.close()
+        except Exception:  # noqa: BLE001 - a failing release in a finalizer is dropped
             pass
 
 

@@ -99,8 +99,36 @@ class _HipBase(IsingSimulation):
         colour = n // 16 if variant == "msc" else n // 2
         return [OBS_SCRATCH_BYTES, 16, colour, colour]
 
+    #: the batch (BatchScan or ReservedBatch) this lattice belongs to, while it does
+    _batch = None
+
     def synchronize(self) -> None:
         _lib.check(self._L.ising_stream_sync(self.stream_ptr), "stream_sync")
+
+    def _after_batch(self) -> None:
+        """Order this simulation's next device work after the last batched sweep of the batch
+        it belongs to: the batch runs on a stream of its own, which the simulation's stream
+        (the null stream by default) does not otherwise wait for."""
+        b = self._batch
+        if b is not None:
+            b.order_after(self.stream_ptr)
+
+    def _own_sweep_allowed(self) -> None:
+        if self._batch is not None:
+            raise RuntimeError("this lattice belongs to a batch: sweep it through the batch "
+                               "(the batch holds its sweep counter) or close the batch first")
+
+    def close(self) -> None:
+        """Wait for this simulation's work and release its device state now (colour arrays and
+        reduction buffers; through the allocator's buffers when one was given). The object is
+        unusable afterwards."""
+        bufs = [getattr(self, "_obs_scratch", None), getattr(self, "_obs_out", None)]
+        bufs += list(getattr(self, "c", None) or []) + list(getattr(self, "w", None) or [])
+        if any(b is not None and getattr(b, "ptr", None) for b in bufs):
+            self.synchronize()
+        for b in bufs:
+            if b is not None:
+                b.free()
 
     def measure_series(self, n_meas: int, every: int = 1) -> np.ndarray:
         """``n_meas`` exact ``(M, E)`` pairs, one after every ``every`` sweeps, as an
@@ -143,6 +171,7 @@ class HipInt8(_HipBase):
         self._table_arr = np.ascontiguousarray(self.table, dtype=np.uint32)
 
     def _split(self, buf, tmp: bool) -> None:
+        self._after_batch()
         _lib.check(self._L.ising_split_int8(buf.ptr, self.c[0].ptr, self.c[1].ptr, self.rows,
                                             self.ly, self.shape[-1], self.stream_ptr),
                    "split_int8")
@@ -162,6 +191,7 @@ class HipInt8(_HipBase):
             _lib.check(self._L.ising_fill_u64(buf.ptr, n, _ONES_INT8, self.stream_ptr), "fill")
 
     def get_lattice(self) -> np.ndarray:
+        self._after_batch()
         full = np.empty(self.shape, dtype=np.int8)
         buf, tmp = self._full_buffer(full.nbytes)
         _lib.check(self._L.ising_merge_int8(self.c[0].ptr, self.c[1].ptr, buf.ptr, self.rows,
@@ -173,6 +203,7 @@ class HipInt8(_HipBase):
         return full
 
     def _half_sweep(self, color: int) -> None:
+        self._own_sweep_allowed()
         own, other = self.c[color], self.c[1 - color]
         _lib.check(self._L.ising_int8_half(
             own.ptr, other.ptr, self.rows, self.lh, self.ly, self.lz, self.ndim,
@@ -183,6 +214,7 @@ class HipInt8(_HipBase):
         n = int(n)
         if n <= 0:
             return
+        self._own_sweep_allowed()
         _lib.check(self._L.ising_int8_sweeps(
             self.c[0].ptr, self.c[1].ptr, self.rows, self.lh, self.ly, self.lz, self.ndim,
             int(self.algorithm == "heatbath"), self._table_arr.ctypes.data,
@@ -190,6 +222,7 @@ class HipInt8(_HipBase):
         self.t += n
 
     def _obs_into(self, out_ptr) -> None:
+        self._after_batch()
         _lib.check(self._L.ising_int8_obs(
             self.c[0].ptr, self.c[1].ptr, self.rows, self.lh, self.ly, self.lz, self.ndim,
             self._obs_scratch.ptr, out_ptr, self.stream_ptr), "int8_obs")
@@ -222,6 +255,7 @@ class HipMultiSpin(_HipBase):
         self.stats_buf = None
 
     def _split(self, buf, tmp: bool) -> None:
+        self._after_batch()
         _lib.check(self._L.ising_split_pack(buf.ptr, self.w[0].ptr, self.w[1].ptr, self.rows,
                                             self.shape[-1], self.stream_ptr), "split_pack")
         self.synchronize()
@@ -240,6 +274,7 @@ class HipMultiSpin(_HipBase):
                        "fill")
 
     def get_lattice(self) -> np.ndarray:
+        self._after_batch()
         full = np.empty(self.shape, dtype=np.int8)
         buf, tmp = self._full_buffer(full.nbytes)
         _lib.check(self._L.ising_unpack_merge(self.w[0].ptr, self.w[1].ptr, buf.ptr, self.rows,
@@ -259,6 +294,7 @@ class HipMultiSpin(_HipBase):
 
     def get_words(self) -> tuple[np.ndarray, np.ndarray]:
         """The two packed colour arrays, (rows, Lx/128) uint64 each."""
+        self._after_batch()
         out = []
         for buf in self.w:
             a = np.empty((self.rows, self.nw), dtype=np.uint64)
@@ -273,6 +309,7 @@ class HipMultiSpin(_HipBase):
         self._run(1, color_only=color)
 
     def _run(self, n: int, color_only: int | None = None) -> None:
+        self._own_sweep_allowed()
         thr4, thr8 = int(self.table[3]), int(self.table[4])
         if color_only is not None:
             own, other = self.w[color_only], self.w[1 - color_only]
@@ -294,6 +331,7 @@ class HipMultiSpin(_HipBase):
         self.t += n
 
     def _obs_into(self, out_ptr) -> None:
+        self._after_batch()
         _lib.check(self._L.ising_msc_obs(self.w[0].ptr, self.w[1].ptr, self.rows, self.nw,
                                          self._obs_scratch.ptr, out_ptr, self.stream_ptr),
                    "msc_obs")

@@ -95,7 +95,50 @@ class Moments:
                 "m2": self.mean(2), "m4": self.mean(4), "binder": self.binder}
 
 
-class BatchScan:
+class _Members:
+    """Membership of simulations in a batch that sweeps them on its own stream.
+
+    While a simulation belongs to a batch, its own sweeps are refused (the batch holds its
+    sweep counter) and its other device work (reading or writing the lattice, observables) is
+    queued behind the batch's last recorded sweep: the batch's stream is a non-blocking stream,
+    which the simulation's stream would otherwise not wait for."""
+
+    _done = None
+    sims: list = []
+
+    def _mark(self) -> None:
+        if self._done is None:
+            self._done = _lib.Event()
+        self._done.record(self.stream.ptr)
+
+    def order_after(self, stream_ptr) -> None:
+        """Work queued on ``stream_ptr`` from now on waits for the batch's last recorded
+        sweep (on the device; the host does not wait)."""
+        if self._done is not None:
+            _lib.check(self._L.ising_stream_wait_event(stream_ptr, self._done.ptr),
+                       "stream_wait_event")
+
+    def _join(self, sims) -> None:
+        for s in sims:
+            if getattr(s, "_batch", None) not in (None, self):
+                raise ValueError("a lattice can belong to one batch at a time")
+        for s in sims:
+            s._batch = self
+
+    def _leave(self, sims) -> None:
+        for s in sims:
+            if getattr(s, "_batch", None) is self:
+                s._batch = None
+                self.order_after(s.stream_ptr)
+
+    def _release_members(self) -> None:
+        self._leave(self.sims)
+        if self._done is not None:
+            self._done.close()
+            self._done = None
+
+
+class BatchScan(_Members):
     """Sweep a list of hip simulations together with one persistent kernel per call.
 
     ``blocks_per_cu``: persistent grid size per CU (None = the occupancy maximum).
@@ -103,9 +146,11 @@ class BatchScan:
     ``persistent``: True = one cooperative launch per ``sweep`` call with a grid barrier between
     half sweeps; False (default) = one launch per half sweep over all lattices plus one
     accumulation launch per sweep (kernel boundaries are cheaper than a grid barrier here).
-    The simulations keep ownership of their device arrays and stay usable (``get_lattice``,
-    ``observables``); their ``t`` advances with every batched sweep. Thresholds are taken at
-    construction: ``set_beta`` on a member afterwards does not reach the batch.
+    The simulations keep ownership of their device arrays; while they belong to the batch (until
+    ``close``) their reads and writes (``get_lattice``, ``observables``, ``set_lattice``) are
+    queued behind the batch's last sweep, their own ``sweep`` is refused, and their ``t``
+    advances with every batched sweep. Thresholds are taken at construction: ``set_beta`` on a
+    member afterwards does not reach the batch.
     """
 
     def __init__(self, sims, blocks_per_cu: int | None = None, msc_mode: int = 0,
@@ -149,8 +194,10 @@ class BatchScan:
         self.tiles_per_workgroup = int(tiles_per_workgroup)
         self.persistent = bool(persistent)
         self.n_spins = np.array([s.n_spins for s in self.sims], dtype=np.int64)
+        self._own_stream = stream is None
         self.stream = stream if stream is not None else _lib.Stream()
         self.graph_sweeps = 0
+        self._join(self.sims)
 
     @classmethod
     def tuned(cls, sims, sweeps: int = 50, use_cache: bool = True) -> "BatchScan":
@@ -193,6 +240,7 @@ class BatchScan:
         if n <= 0:
             return
         _lib.check(self._L.ising_batch_sweeps(self._h, n, self.stream.ptr), "batch_sweeps")
+        self._mark()
         self._advance(n)
 
     def capture(self, sweeps_per_replay: int) -> bool:
@@ -209,6 +257,7 @@ class BatchScan:
             raise RuntimeError("no captured graph: call capture() first")
         _lib.check(self._L.ising_batch_replay(self._h, int(replays), self.stream.ptr),
                    "batch_replay")
+        self._mark()
         self._advance(self.graph_sweeps * int(replays))
 
     def synchronize(self) -> None:
@@ -233,15 +282,22 @@ class BatchScan:
         _lib.check(self._L.ising_batch_reset_stats(self._h, self.stream.ptr), "batch_reset")
 
     def close(self) -> None:
+        """Wait for the batch, release its device tables (and its stream, if it made one);
+        the member simulations leave the batch and may sweep on their own again."""
         if getattr(self, "_h", None):
             self.stream.sync()
             self._L.ising_batch_destroy(self._h)
             self._h = None
+            self._release_members()
+            if getattr(self, "_own_stream", False):
+                self.stream.close()
 
     def __del__(self):
+        if not _lib.releasing_allowed():
+            return
         try:
             self.close()
-        except Exception:  # noqa: BLE001 - interpreter teardown
+        except Exception:  # noqa: BLE001 - a failing release in a finalizer is dropped
             pass
 
 
@@ -261,7 +317,7 @@ def batch_tiles(sims, msc_mode: int = 0) -> int:
     return out.value
 
 
-class ReservedBatch:
+class ReservedBatch(_Members):
     """A batch with fixed device capacity whose lattice list is replaced between sweeps
     (continuous batching). ``assign`` and ``sweep`` are stream-ordered on ``stream`` and allocate
     nothing; a sweep queued before an ``assign`` still runs the old list.
@@ -293,6 +349,9 @@ class ReservedBatch:
         desc = _describe_all(sims, limits)
         _lib.check(self._L.ising_batch_assign(self._h, desc, len(sims), self.stream.ptr),
                    "batch_assign")
+        self._mark()
+        self._leave([s for s in self.sims if all(s is not t for t in sims)])
+        self._join(sims)
         self.sims = sims
         self._left = [int(n) if n else None for n in (limits or [0] * len(sims))]
 
@@ -301,6 +360,7 @@ class ReservedBatch:
         if n <= 0 or not self.sims:
             return
         _lib.check(self._L.ising_batch_sweeps(self._h, n, self.stream.ptr), "batch_sweeps")
+        self._mark()
         for i, s in enumerate(self.sims):
             left = self._left[i]
             if left is None:
@@ -319,11 +379,14 @@ class ReservedBatch:
             self.stream.sync()
             self._L.ising_batch_destroy(self._h)
             self._h = None
+            self._release_members()
 
     def __del__(self):
+        if not _lib.releasing_allowed():
+            return
         try:
             self.close()
-        except Exception:  # noqa: BLE001 - interpreter teardown
+        except Exception:  # noqa: BLE001 - a failing release in a finalizer is dropped
             pass
 
 

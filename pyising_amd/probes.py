@@ -18,12 +18,31 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import hashlib
 import json
 import sys
+from pathlib import Path
 
 import numpy as np
 
 from . import _lib
+
+
+#: sources the msc half-sweep kernel is compiled from (a PMC summary records their digest, so
+#: a reader can tell whether it was taken on the kernel built now)
+MSC_KERNEL_SOURCES = ("common.hpp", "msc_core.hpp", "msc_kernels.hip")
+
+
+def kernel_source_digest(names=MSC_KERNEL_SOURCES) -> str:
+    """First 16 hex digits of sha256 over the named csrc files (in order) and the compile
+    command's flags."""
+    from .build import CSRC, command
+
+    h = hashlib.sha256()
+    for n in names:
+        h.update((CSRC / n).read_bytes())
+    h.update(" ".join(a for a in command(Path("x.so")) if a.startswith("-")).encode())
+    return h.hexdigest()[:16]
 
 
 def device_info() -> dict:
@@ -48,8 +67,10 @@ def copy_bandwidth(nbytes: int = 4 << 30, repeats: int = 20, blocks: int = 1024)
         _lib.check(lib.ising_copy(b.ptr, a.ptr, nbytes, blocks, None))
         e1.record()
         rates.append(2 * nbytes / (e0.elapsed_ms(e1) * 1e-3) / 1e9)
-    a.free()
-    b.free()
+    for h in (a, b):
+        h.free()
+    for ev in (e0, e1):
+        ev.close()
     return float(np.median(rates))
 
 
@@ -66,6 +87,9 @@ def philox_rate(calls: int = 512, repeats: int = 7, blocks_per_cu: int = 32) -> 
         _lib.check(lib.ising_philox_rate(out.ptr, blocks, calls, 1, 2, None))
         e1.record()
         rates.append(blocks * 256 * calls / (e0.elapsed_ms(e1) * 1e-3) / 1e9)
+    out.free()
+    for ev in (e0, e1):
+        ev.close()
     return float(np.median(rates))
 
 
@@ -77,6 +101,8 @@ def _run_copy(mib: int, reps: int) -> dict:
     for _ in range(reps):
         _lib.check(lib.ising_copy(b.ptr, a.ptr, n, 1024, None))
     _lib.device_sync()
+    a.free()
+    b.free()
     return {"probe": "copy", "bytes_read_per_launch": n, "bytes_written_per_launch": n,
             "launches": reps}
 
@@ -88,7 +114,7 @@ def _run_msc(L: int, sweeps: int, mode: int, k: int) -> dict:
     sim = HipMultiSpin((L, L), BETA_C_2D, seed=1)
     sim.mode, sim.tile_k = mode, k
     sim.sweep(sweeps)
-    sim.synchronize()
+    sim.close()
     return {"probe": "msc", "L": L, "sweeps": sweeps, "launches": 2 * sweeps,
             "algorithmic_bytes_per_launch": int(0.375 * L * L / 2)}
 
@@ -100,7 +126,7 @@ def _run_int8(L: int, sweeps: int, dim: int, algorithm: str) -> dict:
     beta = BETA_C_2D if dim == 2 else BETA_C_3D
     sim = HipInt8((L,) * dim, beta, seed=1, algorithm=algorithm)
     sim.sweep(sweeps)
-    sim.synchronize()
+    sim.close()
     return {"probe": "int8", "L": L, "dim": dim, "sweeps": sweeps, "launches": 2 * sweeps,
             "algorithmic_bytes_per_launch": int(3.0 * L**dim / 2)}
 
@@ -116,6 +142,8 @@ def _run_batch(sweeps: int, msc_tile: str, moments: bool) -> dict:
     scan.sweep(sweeps)
     scan.synchronize()
     scan.close()
+    for sim in sims:
+        sim.close()
     return {"probe": "batch", "lattices": len(sims), "spins": int(sum(s.n_spins for s in sims)),
             "sweeps": sweeps, "msc_tile": msc_tile, "moments": moments}
 
@@ -151,6 +179,7 @@ def main(argv=None) -> int:
     ph.add_argument("--calls", type=int, default=512)
     ph.add_argument("--reps", type=int, default=3)
     sub.add_parser("ceilings")
+    sub.add_parser("digest")
     b = sub.add_parser("batch")
     b.add_argument("--sweeps", type=int, default=50)
     b.add_argument("--msc-tile", default="lockstep", choices=("lockstep", "queue"))
@@ -172,6 +201,8 @@ def main(argv=None) -> int:
         out = _run_batch(a.sweeps, a.msc_tile, not a.no_moments)
     elif a.cmd == "serve":
         out = _run_serve(a.requests, a.rate)
+    elif a.cmd == "digest":
+        out = {"kernel_sources_sha16": kernel_source_digest()}
     else:
         out = {"copy_bandwidth_GBs": copy_bandwidth(), "philox_calls_per_ns": philox_rate(),
                **device_info()}

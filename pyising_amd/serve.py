@@ -377,10 +377,14 @@ class Server:
         self.batch = ReservedBatch(max_slots, max_tiles, self.work, msc_mode=msc_mode)
 
     def close(self) -> None:
+        """Release the batch, the streams and the device memory now (explicitly, not at
+        interpreter shutdown)."""
         self.batch.close()
         self.setup.sync()
         self._mem.free()
         self.staging.free()
+        self.setup.close()
+        self.work.close()
 
     def run(self, requests, realtime: bool = True, timeout: float = 600.0,
             keep_lattices: bool = False) -> ServeReport:
@@ -464,4 +468,5 @@ class Server:
                 active[i].remaining -= min(step, active[i].remaining)
         self.setup.sync()
         rep.wall = clock()
+        step_ev.close()
         return rep

@@ -1,6 +1,7 @@
 """Writes the golden fixtures of this directory (run from the repository root on a CPU):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            (known answers and end states)
+    python tests/golden/make_golden.py --long     (full-length runs of the configurations)
 
 * ``reference_known_answers.json`` -- the known-answer data the reference's own tests hold,
   each entry citing the test it comes from: the Random123 Philox4x32-10 vectors
@@ -15,6 +16,13 @@
   (tests/test_backends.py:22-67), config-shaped small cases, and config-sized cases kept as a
   sha256 of the lattice plus exact (M, E). The lattice digest is sha256 over the full int8
   lattice (+1 / -1) in C order, the layout ``get_lattice()`` returns.
+
+* ``oracle_long_runs.json`` -- the configured workloads at their full lengths, from the C
+  restatement: msc L=16384 at T_c (seed 1, hot start) for 1000 sweeps and int8 L=8192 at
+  T = 1.5, 2.269, 3.0 for 1000 sweeps, with checkpoints (sha256 + exact M, E) along the way;
+  3D int8 L=256 (seed 7) for 2000 sweeps, Metropolis and heat bath, hot and cold starts at K_c
+  and hot starts at K = 0.20, 0.24, with the exact (M, E) series every 10 sweeps and the final
+  sha256.
 
 The oracle is pinned independently of these files (tests/test_oracle_*.py); the fixtures let
 the GPU tests check end states without running the oracle, and catch any later change of the
@@ -187,7 +195,49 @@ def end_states() -> list[dict]:
     return out
 
 
+def long_run(name, variant, shape, beta, seed, sweeps, every, algorithm="metropolis",
+             init="hot", lattice_every=None):
+    """A full-length run of the C restatement: exact (M, E) after every ``every`` sweeps and the
+    lattice digest after every ``lattice_every`` sweeps (default: the end only)."""
+    t0 = time.time()
+    cls = CO.CInt8 if variant == "int8" else CO.CMsc
+    sim = cls(tuple(shape), beta, seed=seed, algorithm=algorithm, init=init)
+    lattice_every = lattice_every or sweeps
+    series, digests = [], {}
+    done = 0
+    while done < sweeps:
+        sim.sweep(every)
+        done += every
+        series.append([int(x) for x in sim.observables_exact()])
+        if done % lattice_every == 0:
+            digests[str(done)] = digest(sim.get_lattice())
+    print(f"{name}: {sweeps} sweeps, M, E = {series[-1]} ({time.time() - t0:.0f} s)", flush=True)
+    return {"name": name, "variant": variant, "shape": list(shape), "beta": beta, "seed": seed,
+            "algorithm": algorithm, "init": init, "sweeps": sweeps, "every": every,
+            "series": series, "sha256_at": digests}
+
+
+def long_runs() -> list[dict]:
+    out = [long_run("msc_L16384_Tc_hot_1000", "msc", (16384, 16384), BETA_C_2D, 1, 1000, 100,
+                    lattice_every=100)]
+    for T in (1.5, 2.269, 3.0):
+        out.append(long_run(f"int8_2d_L8192_T{T}_hot_1000", "int8", (8192, 8192), 1.0 / T, 1,
+                            1000, 100, lattice_every=250))
+    for alg in ("metropolis", "heatbath"):
+        for K, init in ((BETA_C_3D, "hot"), (BETA_C_3D, "cold"), (0.20, "hot"), (0.24, "hot")):
+            out.append(long_run(f"int8_3d_L256_K{K}_{alg}_{init}_2000", "int8", (256, 256, 256),
+                                K, 7, 2000, 10, alg, init))
+    return out
+
+
 def main() -> int:
+    if "--long" in sys.argv[1:]:
+        data = {"numpy": np.__version__, "engine": "oracle C restatement (oracle/ising_oracle.c)",
+                "note": "series: exact (M, E) after every `every` sweeps; sha256_at: digest of "
+                        "the full int8 lattice (+1/-1, C order) after that many sweeps",
+                "cases": long_runs()}
+        (HERE / "oracle_long_runs.json").write_text(json.dumps(data) + "\n")
+        return 0
     ka = known_answers()
     (HERE / "reference_known_answers.json").write_text(json.dumps(ka, indent=1) + "\n")
     states = {"numpy": np.__version__,
