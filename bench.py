@@ -28,7 +28,7 @@ sys.path.insert(0, str(ROOT))
 L_HEADLINE = 16384
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = 0.375
-PMC_FILE = ROOT / "profiles" / "msc_half_L16384_pmc.json"
+PMC_FILE = ROOT / "profiles" / "msc_L16384_pmc.json"
 
 
 def cpu_model() -> str:

@@ -124,8 +124,12 @@ int ising_int8_obs(const int8_t* d_c0, const int8_t* d_c1, long long rows, int l
  * Philox calls per word, then packed one-call rounds for the undecided words; R in {2,4} x 256
  * words per workgroup), 1 lockstep, 2 eager (all 16 calls; the msc-eager32 ablation of
  * cupy_backend.py:176-180), 3 by size (default, k = 42: lockstep up to 2^18 words per colour,
- * e.g. 4096 x 8192, queue compaction with k above). All modes give identical words. No reference
- * counterpart (kernel selection). */
+ * e.g. 4096 x 8192, queue compaction with k above, fused as in mode 4 in ising_msc_sweeps_sync
+ * where it applies and a half sweep has at least 2048 tiles, e.g. 8192 x 16384), 4 fused queue
+ * compaction (k = 10 * cap + 2, cap in 3..5: the
+ * compaction tiles of up to 64 half sweeps in one launch, ordered by per-tile completion
+ * counters; only in ising_msc_sweeps_sync, a single half sweep runs mode 0). All modes give
+ * identical words. No reference counterpart (kernel selection). */
 int ising_msc_config(int mode, int k);
 /* The concrete kernel (mode 0, 1 or 2, and k for mode 0, else 0) that a half sweep of a
  * rows x (128 nw) lattice runs with `mode`, `k` (mode < 0: the process default). Host only.
@@ -150,6 +154,30 @@ int ising_msc_sweeps_ex(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw, 
                         long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4,
                         uint32_t thr8, int mode, int k, unsigned long long* d_stats,
                         void* stream);
+/* n_sweeps full sweeps like ising_msc_sweeps_ex, with a sync workspace for the fused form
+ * (mode 4, or mode 3 / the default where it applies: Lx/128 in {64, 128, 256, 512}, a
+ * power-of-two number >= 16 of 512-word tiles, and the device places workgroups round-robin
+ * over its 8 XCDs or all on one). d_sync = ising_msc_sync_bytes of device memory, zero before its
+ * first use and kept by the caller between calls together with *epoch (host, in/out: the
+ * workspace's counter value, 0 for a zeroed workspace; advanced by the half sweeps run fused).
+ * The first call on a device probes its workgroup placement once (synchronous). A failed
+ * dependency wait leaves the lattice unswept from that tile on and sets a failure bit that
+ * ising_msc_sync_status reports; the workspace must then be zeroed (and *epoch reset to 0).
+ * Replaces IsingSimulation.sweep (base.py:85-90) for msc with one launch per up to 32 sweeps
+ * instead of one per half sweep. */
+int ising_msc_sweeps_sync(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw, uint32_t t0,
+                          long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4,
+                          uint32_t thr8, int mode, int k, uint32_t* d_sync, uint32_t* epoch,
+                          unsigned long long* d_stats, void* stream);
+/* Bytes of the sync workspace of a rows x (128 nw) lattice (host only). No reference
+ * counterpart. */
+int ising_msc_sync_bytes(long long rows, int nw, size_t* bytes);
+/* Failure bits of the sync workspace of a rows x (128 nw) lattice whose counters should all
+ * hold `epoch` (synchronous read on `stream`): 0 = none, 1 = a dependency wait gave up,
+ * 2 = a tile counter differs from epoch (a tile was skipped: workgroups were not placed
+ * round-robin over the XCDs). No reference counterpart. */
+int ising_msc_sync_status(const uint32_t* d_sync, long long rows, int nw, uint32_t epoch,
+                          int* failed, void* stream);
 /* Exact (M, E) of a packed lattice: observables_packed, observables_torch.py:85-111.
  * d_scratch2 = ISING_OBS_SCRATCH_BYTES of device memory (any contents). */
 int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, int nw,

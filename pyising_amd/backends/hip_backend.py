@@ -24,8 +24,8 @@ from .base import IsingSimulation
 _ONES_INT8 = 0x0101010101010101
 _ALL_ONES = 0xFFFFFFFFFFFFFFFF
 
-#: msc kernel modes of ising_msc_sweeps_ex
-MSC_DEFAULT, MSC_COMPACT, MSC_LOCKSTEP, MSC_EAGER, MSC_BY_SIZE = -1, 0, 1, 2, 3
+#: msc kernel modes of ising_msc_sweeps_ex / ising_msc_sweeps_sync
+MSC_DEFAULT, MSC_COMPACT, MSC_LOCKSTEP, MSC_EAGER, MSC_BY_SIZE, MSC_FUSED = -1, 0, 1, 2, 3, 4
 #: ISING_OBS_SCRATCH_BYTES of ising_hip.h: partial sums of the observable reductions
 OBS_SCRATCH_BYTES = 16384
 
@@ -253,6 +253,50 @@ class HipMultiSpin(_HipBase):
         self.n_words = self.rows * self.nw
         self.w = [self._alloc(8 * self.n_words), self._alloc(8 * self.n_words)]
         self.stats_buf = None
+        self._sync = None  # workspace of the fused sweeps, made at the first sweep
+        self._epoch = C.c_uint32(0)
+        self._sync_unchecked = False
+
+    def _sync_buffer(self):
+        """The fused sweeps' workspace (per-tile completion counters), zeroed once; its counter
+        value travels with it in ``_epoch``."""
+        if self._sync is None:
+            n = C.c_size_t()
+            _lib.check(self._L.ising_msc_sync_bytes(self.rows, self.nw, C.byref(n)), "sync_bytes")
+            self._sync = _lib.DeviceBuffer(n.value)
+            _lib.check(self._L.ising_fill_u64(self._sync.ptr, n.value // 8, 0, self.stream_ptr),
+                       "fill")
+            self._epoch.value = 0
+        return self._sync
+
+    def check_sync(self) -> None:
+        """Raise if a fused launch since the last check failed (a bounded dependency wait
+        gave up, or a tile's counter is not where every counter should be); the lattice is then
+        not a valid continuation and the workspace is zeroed again. Synchronous."""
+        if not self._sync_unchecked or self._sync is None:
+            return
+        failed = C.c_int()
+        _lib.check(self._L.ising_msc_sync_status(self._sync.ptr, self.rows, self.nw,
+                                                 self._epoch.value, C.byref(failed),
+                                                 self.stream_ptr), "sync_status")
+        self._sync_unchecked = False
+        if failed.value:
+            n = self._sync.nbytes
+            _lib.check(self._L.ising_fill_u64(self._sync.ptr, n // 8, 0, self.stream_ptr), "fill")
+            self._epoch.value = 0
+            raise _lib.HipError(f"fused msc sweep failed (status {failed.value}): the lattice "
+                                "is invalid; sweep with mode=MSC_COMPACT instead")
+
+    def synchronize(self) -> None:
+        super().synchronize()
+        self.check_sync()
+
+    def close(self) -> None:
+        try:
+            super().close()  # waits, and reports a fused failure not yet checked
+        finally:
+            if self._sync is not None:
+                self._sync.free()
 
     def _split(self, buf, tmp: bool) -> None:
         self._after_batch()
@@ -282,6 +326,7 @@ class HipMultiSpin(_HipBase):
         buf.download(full, self.stream_ptr)
         if tmp:
             buf.free()
+        self.check_sync()
         return full
 
     def kernel(self) -> dict:
@@ -289,7 +334,8 @@ class HipMultiSpin(_HipBase):
         mode, k = C.c_int(), C.c_int()
         _lib.check(self._L.ising_msc_kernel(self.rows, self.nw, self.mode, self.tile_k,
                                             C.byref(mode), C.byref(k)), "msc_kernel")
-        names = {MSC_COMPACT: "queue compaction", MSC_LOCKSTEP: "lockstep", MSC_EAGER: "eager"}
+        names = {MSC_COMPACT: "queue compaction", MSC_LOCKSTEP: "lockstep", MSC_EAGER: "eager",
+                 MSC_FUSED: "fused queue compaction"}
         return {"mode": mode.value, "name": names[mode.value], "k": k.value}
 
     def get_words(self) -> tuple[np.ndarray, np.ndarray]:
@@ -318,10 +364,12 @@ class HipMultiSpin(_HipBase):
                 self.k0, self.k1, thr4, thr8, self.mode, self.tile_k, self._stats_ptr(),
                 self.stream_ptr), "msc_half")
             return
-        _lib.check(self._L.ising_msc_sweeps_ex(
+        sync = self._sync_buffer()
+        _lib.check(self._L.ising_msc_sweeps_sync(
             self.w[0].ptr, self.w[1].ptr, self.rows, self.nw, self.t & 0xFFFFFFFF, n,
-            self.k0, self.k1, thr4, thr8, self.mode, self.tile_k, self._stats_ptr(),
-            self.stream_ptr), "msc_sweeps")
+            self.k0, self.k1, thr4, thr8, self.mode, self.tile_k, sync.ptr,
+            C.byref(self._epoch), self._stats_ptr(), self.stream_ptr), "msc_sweeps")
+        self._sync_unchecked = True
 
     def sweep(self, n: int = 1) -> None:
         n = int(n)
@@ -338,7 +386,9 @@ class HipMultiSpin(_HipBase):
 
     def observables_exact(self) -> tuple[int, int]:
         self._obs_into(self._obs_out.ptr)
-        return self._read_obs()
+        out = self._read_obs()
+        self.check_sync()
+        return out
 
     def philox_stats(self, sweeps: int = 2) -> dict:
         """Philox work of ``sweeps`` sweeps (diagnostic; advances the simulation).

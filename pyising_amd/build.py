@@ -30,26 +30,49 @@ def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
+OBJ_DIR = HERE.parent / "_build"
+
+FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # the device code's LDS atomics are issued by one lane with a wave-uniform address
+         # (queue appends); the atomic optimizer's per-lane scan around them is pure overhead
+         "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]
+
+
 def command(out: Path = OUT) -> list[str]:
-    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-            "-Wall", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            # the device code's LDS atomics are issued by one lane with a wave-uniform address
-            # (queue appends); the atomic optimizer's per-lane scan around them is pure overhead
-            "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
-            f"-I{CSRC}", f"-I{INCLUDE}", "-o", str(out)] + [str(s) for s in sources()]
+    """The one-step form of the build (every source into the library), for reference; build()
+    compiles the sources in parallel with the same flags and links them."""
+    return [hipcc(), *FLAGS, "-shared", f"-I{CSRC}", f"-I{INCLUDE}", "-o", str(out)] + \
+        [str(s) for s in sources()]
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every source (always; a build is never skipped) and replace the library."""
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = command(tmp)
+def _run(cmd: list[str]) -> str:
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"hipcc failed ({res.returncode}): {' '.join(cmd)}")
+    return res.stdout + res.stderr
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every source (always; a build is never skipped), one hipcc per source in
+    parallel, link them and replace the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    OBJ_DIR.mkdir(exist_ok=True)
+    srcs = sources()
+    objs = [OBJ_DIR / (s.stem + ".o") for s in srcs]
+    cmds = [[hipcc(), *FLAGS, f"-I{CSRC}", f"-I{INCLUDE}", "-c", "-o", str(o), str(s)]
+            for s, o in zip(srcs, objs)]
+    workers = max(1, min(len(cmds), os.cpu_count() or 1))
+    with ThreadPoolExecutor(workers) as pool:
+        logs = list(pool.map(_run, cmds))
+    tmp = OUT.with_suffix(".so.tmp")
+    logs.append(_run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-o", str(tmp)] +
+                     [str(o) for o in objs]))
     if verbose:
         noise = "packed-fp32-ops"
-        msg = "\n".join(ln for ln in (res.stdout + res.stderr).splitlines() if noise not in ln)
+        msg = "\n".join(ln for log in logs for ln in log.splitlines() if noise not in ln)
         if msg.strip():
             sys.stderr.write(msg + "\n")
     os.replace(tmp, OUT)

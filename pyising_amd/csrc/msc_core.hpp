@@ -75,6 +75,17 @@ struct MscRowLoad {
     u32 right;  // 1: colour sites at odd columns, horizontal neighbour j + 1 (uniform)
 };
 
+// Load of one packed word. COH: a device-scope (agent-coherent) load, which always misses in
+// the CU's L1: for kernels in which another workgroup of the same launch may have rewritten
+// the word after this CU cached it (the fused sweeps); otherwise an ordinary cached load.
+template <bool COH>
+__device__ __forceinline__ u64 msc_ld(const u64* p)
+{
+    if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+
+template <bool COH = false>
 __device__ __forceinline__ MscRowLoad msc_load_row(const MscArgs& a, u32 w0, u32 lane)
 {
     w0 = uni(w0);
@@ -84,17 +95,17 @@ __device__ __forceinline__ MscRowLoad msc_load_row(const MscArgs& a, u32 w0, u32
     const u32 yd = (y == 0u) ? a.rows - 1u : y - 1u;
     const u64* crow = a.other + (u64)y * a.nw;
     MscRowLoad L;
-    L.o = (a.own + w0)[lane];
-    L.c = (crow + x0)[lane];
-    L.up = (a.other + (u64)yu * a.nw + x0)[lane];
-    L.dn = (a.other + (u64)yd * a.nw + x0)[lane];
+    L.o = msc_ld<COH>(a.own + w0 + lane);
+    L.c = msc_ld<COH>(crow + x0 + lane);
+    L.up = msc_ld<COH>(a.other + (u64)yu * a.nw + x0 + lane);
+    L.dn = msc_ld<COH>(a.other + (u64)yd * a.nw + x0 + lane);
     L.right = (y + a.color) & 1u;
     if (L.right) {
         const u32 xe = (x0 + 64u == a.nw) ? 0u : x0 + 64u;  // word x0 + 64, wrapped
-        L.e = (u32)crow[xe];
+        L.e = (u32)msc_ld<COH>(crow + xe);
     } else {
         const u32 xe = (x0 == 0u) ? a.nw - 1u : x0 - 1u;  // word x0 - 1, wrapped
-        L.e = (u32)(crow[xe] >> 32);
+        L.e = (u32)(msc_ld<COH>(crow + xe) >> 32);
     }
     return L;
 }
@@ -447,9 +458,10 @@ struct MscTileLoads {
 
 // Branch-free: words past the end load a clamped (valid) address instead, and their results
 // are discarded by the caller.
-template <int R, int LAY>
+template <int R, int LAY, bool COH = false>
 __device__ __forceinline__ void msc_tile_load(const MscArgs& a, u32 base, MscTileLoads<R, LAY>& T)
 {
+    static_assert(!COH || LAY == MSC_ROWW, "coherent loads are implemented for whole-row waves");
     const u32 tid = threadIdx.x;
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
@@ -457,7 +469,7 @@ __device__ __forceinline__ void msc_tile_load(const MscArgs& a, u32 base, MscTil
             // n_words is a multiple of 64: clamp the wave's first word
             const u32 w0 = base + (u32)rr * 256u + (tid & ~63u);
             const u32 wc = w0 < a.n_words ? w0 : a.n_words - 64u;
-            if (LAY == MSC_ROWW) T.lr[rr] = msc_load_row(a, wc, tid & 63u);
+            if (LAY == MSC_ROWW) T.lr[rr] = msc_load_row<COH>(a, wc, tid & 63u);
             else T.ls[rr] = msc_load_seg(a, wc, tid & 63u);
         } else {
             const u32 w = base + (u32)rr * 256u + tid;
@@ -470,11 +482,32 @@ struct MscNoPrefetch {
     __device__ void operator()() const {}
 };
 
-template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, class PF>
+// Phase A's Philox calls of a tile, computed by the caller ahead of the tile (PRE): level
+// pairs 0 .. CAP-1 of the word slot r * 256 + threadIdx.x for every round r.
+template <int R, int CAP>
+struct MscPhaseACalls {
+    PhiloxOut r[R][CAP];
+};
+
+template <int R, int CAP>
+__device__ __forceinline__ void msc_phase_a_calls(const MscArgs& a, u32 base, int rr,
+                                                  MscPhaseACalls<R, CAP>& P)
+{
+    const MscCounter ctr = msc_counter(base + (u32)rr * 256u + threadIdx.x);
+#pragma unroll
+    for (int lp = 0; lp < CAP; ++lp) {
+        P.r[rr][lp] = msc_philox(a, ctr, (u32)lp);
+        // pinned where it is written, so that it covers the memory latency that follows
+        asm volatile("" : "+v"(P.r[rr][lp].x), "+v"(P.r[rr][lp].y), "+v"(P.r[rr][lp].z), "+v"(P.r[rr][lp].w));
+    }
+}
+
+template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, class PF, bool PRE = false>
 __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes, u32 base,
                                                      MscQueueLds<R>& L,
                                                      const MscTileLoads<R, LAY>& T, PF prefetch,
-                                                     u32& calls, u32& slots, u32& words)
+                                                     u32& calls, u32& slots, u32& words,
+                                                     const MscPhaseACalls<R, CAP>* pre = nullptr)
 {
     static_assert(CAP >= 1 && CAP <= 15, "phase A levels");
     const u32 tid = threadIdx.x;
@@ -493,14 +526,19 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
         const bool valid = w < a.n_words;
         // the CAP Philox calls depend on the counter only: computed before the spin loads are
         // consumed, they cover the memory latency (at the start of every workgroup above all)
-        const MscCounter ctr = msc_counter(w);
         PhiloxOut r[CAP];
+        if constexpr (PRE) {
 #pragma unroll
-        for (int lp = 0; lp < CAP; ++lp) {
-            r[lp] = msc_philox(a, ctr, (u32)lp);
-            // an empty asm that "modifies" the outputs pins the call here: the optimiser would
-            // otherwise sink it below the classification and its wait for the loads
-            asm volatile("" : "+v"(r[lp].x), "+v"(r[lp].y), "+v"(r[lp].z), "+v"(r[lp].w));
+            for (int lp = 0; lp < CAP; ++lp) r[lp] = pre->r[rr][lp];
+        } else {
+            const MscCounter ctr = msc_counter(w);
+#pragma unroll
+            for (int lp = 0; lp < CAP; ++lp) {
+                r[lp] = msc_philox(a, ctr, (u32)lp);
+                // an empty asm that "modifies" the outputs pins the call here: the optimiser
+                // would otherwise sink it below the classification and its wait for the loads
+                asm volatile("" : "+v"(r[lp].x), "+v"(r[lp].y), "+v"(r[lp].z), "+v"(r[lp].w));
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
         MscState s;

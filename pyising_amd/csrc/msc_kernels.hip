@@ -130,15 +130,25 @@ constexpr u32 MSC_LOCKSTEP_MAX_WORDS = 1u << 18;
 // never reads a mode without its k while another thread calls ising_msc_config.
 static std::atomic<int> g_msc_default{(3 << 8) | 42};
 
-// the concrete mode (0, 1 or 2) and k for a lattice of n_words words per colour
-static void msc_resolve(int* mode, int* k, u32 n_words)
+// fused sweeps (msc_fused.hip): 0 not applicable, 1 applicable, 2 applicable and the default's
+// choice for this size
+int msc_fused_applies(long long rows, int nw);
+int msc_fused_sweeps(u64* w0, u64* w1, long long rows, int nw, u32 t0, long long n_sweeps, u32 k0,
+                     u32 k1, u32 thr4, u32 thr8, int cap, u32* sync, u32* epoch,
+                     unsigned long long* stats, hipStream_t st);
+
+// the concrete mode (0, 1 or 2, or 4 for sweeps with a sync workspace) and k for a lattice of
+// n_words words per colour; fused = msc_fused_applies() for the lattice, 0 where the call has
+// no sync workspace
+static void msc_resolve(int* mode, int* k, u32 n_words, int fused = 0)
 {
     if (*mode < 0) {
         const int d = g_msc_default.load(std::memory_order_relaxed);
         *mode = d >> 8;
         *k = d & 0xFF;
     }
-    if (*mode == 3) *mode = n_words <= MSC_LOCKSTEP_MAX_WORDS ? 1 : 0;
+    if (*mode == 3) *mode = n_words <= MSC_LOCKSTEP_MAX_WORDS ? 1 : (fused == 2 ? 4 : 0);
+    if (*mode == 4 && !fused) *mode = 0;  // a single half sweep: the same tiles, one launch
 }
 
 static int msc_check(const void* own, const void* other, long long rows, int nw)
@@ -154,13 +164,15 @@ static int msc_check(const void* own, const void* other, long long rows, int nw)
 static int msc_mode_ok(int mode, int k)
 {
     if (mode < 0) return ISING_OK;  // process default
-    if (mode > 3)
-        return ising_fail(ISING_EINVAL, "msc: mode must be 0 (queue compaction), 1 (lockstep), 2 (eager) or 3 (by size)");
+    if (mode > 4)
+        return ising_fail(ISING_EINVAL, "msc: mode must be 0 (queue compaction), 1 (lockstep), 2 (eager), 3 (by size) or 4 (fused queue compaction)");
     if (mode == 0 || mode == 3) {
         const int cap = k / 10, rr = k % 10;
         if (cap < 3 || cap > 6 || (rr != 2 && rr != 4))
             return ising_fail(ISING_EINVAL, "msc: queue compaction k = 10 * cap + R with cap in 3..6, R in {2,4}");
     }
+    if (mode == 4 && (k % 10 != 2 || k / 10 < 3 || k / 10 > 5))
+        return ising_fail(ISING_EINVAL, "msc: fused queue compaction k = 10 * cap + 2 with cap in 3..5");
     return ISING_OK;
 }
 
@@ -257,9 +269,28 @@ static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32
 
 static int msc_sweeps_impl(u64* w0, u64* w1, long long rows, int nw, u32 t0, long long n_sweeps,
                            u32 k0, u32 k1, u32 thr4, u32 thr8, int mode, int k,
-                           unsigned long long* stats, hipStream_t st)
+                           unsigned long long* stats, hipStream_t st, u32* sync = nullptr,
+                           u32* epoch = nullptr)
 {
     if (n_sweeps < 0) return ising_fail(ISING_EINVAL, "msc_sweeps: negative sweep count");
+    if (sync) {
+        int rc = msc_check(w0, w1, rows, nw);
+        if (rc) return rc;
+        if ((rc = msc_mode_ok(mode, k))) return rc;
+        const int fused = msc_fused_applies(rows, nw);
+        if (mode == 4 && !fused)
+            return ising_fail(ISING_EINVAL, "msc: the fused form needs Lx/128 in {64, 128, 256, 512}, a power-of-two number of 512-word tiles (>= 16) and round-robin or single-XCD placement");
+        int m = mode, kk = k;
+        msc_resolve(&m, &kk, (u32)(rows * nw), fused);
+        if (m == 4) {
+            if (n_sweeps == 0) return ISING_OK;
+            const int cap = (mode == 4) ? kk / 10 : 4;
+            return msc_fused_sweeps(w0, w1, rows, nw, t0, n_sweeps, k0, k1, thr4, thr8, cap, sync,
+                                    epoch, stats, st);
+        }
+        mode = m;
+        k = kk;
+    }
     for (long long s = 0; s < n_sweeps; ++s) {
         const u32 t = t0 + (u32)s;
         int rc = msc_half_impl(w0, w1, rows, nw, t, 0u, k0, k1, thr4, thr8, mode, k, stats, st);
@@ -292,10 +323,20 @@ int ising_msc_kernel(long long rows, int nw, int mode, int k, int* out_mode, int
         return ising_fail(ISING_EINVAL, "msc_kernel: bad geometry");
     int rc = msc_mode_ok(mode, k);
     if (rc) return rc;
-    msc_resolve(&mode, &k, (u32)(rows * nw));
+    msc_resolve(&mode, &k, (u32)(rows * nw), msc_fused_applies(rows, nw));
     *out_mode = mode;
-    *out_k = mode == 0 ? k : 0;
+    *out_k = (mode == 0 || mode == 4) ? (mode == 4 && k % 10 != 2 ? 42 : k) : 0;
     return ISING_OK;
+}
+
+int ising_msc_sweeps_sync(uint64_t* w0, uint64_t* w1, long long rows, int nw, uint32_t t0,
+                          long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4,
+                          uint32_t thr8, int mode, int k, uint32_t* d_sync, uint32_t* epoch,
+                          unsigned long long* stats, void* stream)
+{
+    if (!d_sync || !epoch) return ising_fail(ISING_EINVAL, "msc_sweeps_sync: null workspace or epoch");
+    return msc_sweeps_impl(w0, w1, rows, nw, t0, n_sweeps, k0, k1, thr4, thr8, mode, k, stats,
+                           as_stream(stream), d_sync, epoch);
 }
 
 int ising_msc_half(uint64_t* own, const uint64_t* other, long long rows, int nw, uint32_t t,

@@ -30,18 +30,18 @@ from . import _lib
 
 #: sources the msc half-sweep kernel is compiled from (a PMC summary records their digest, so
 #: a reader can tell whether it was taken on the kernel built now)
-MSC_KERNEL_SOURCES = ("common.hpp", "msc_core.hpp", "msc_kernels.hip")
+MSC_KERNEL_SOURCES = ("common.hpp", "msc_core.hpp", "msc_kernels.hip", "msc_fused.hip")
 
 
 def kernel_source_digest(names=MSC_KERNEL_SOURCES) -> str:
     """First 16 hex digits of sha256 over the named csrc files (in order) and the compile
     command's flags."""
-    from .build import CSRC, command
+    from .build import CSRC, FLAGS
 
     h = hashlib.sha256()
     for n in names:
         h.update((CSRC / n).read_bytes())
-    h.update(" ".join(a for a in command(Path("x.so")) if a.startswith("-")).encode())
+    h.update(" ".join(FLAGS).encode())
     return h.hexdigest()[:16]
 
 

@@ -66,8 +66,13 @@ def test_argument_validation_needs_no_gpu():
     for k in (43, 72, 22, 41, 151):
         rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 0, k, None, None)
         assert rc == -1 and b"cap" in lib.ising_last_error()
-    rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 4, 42, None, None)
+    rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 5, 42, None, None)
     assert rc == -1 and b"mode" in lib.ising_last_error()
+    for k in (44, 62, 22):  # the fused form has 512-word tiles (R = 2) and cap 3..5
+        rc = lib.ising_msc_half_ex(1, 1, 16, 1, 0, 0, 0, 0, 0, 0, 4, k, None, None)
+        assert rc == -1 and b"fused" in lib.ising_last_error()
+    rc = lib.ising_msc_sweeps_sync(1, 1, 16, 1, 0, 1, 0, 0, 0, 0, 4, 42, None, None, None, None)
+    assert rc == -1 and b"workspace" in lib.ising_last_error()
     rc = lib.ising_msc_half_ex(1, 1, 15, 1, 0, 0, 0, 0, 0, 0, 0, 42, None, None)
     assert rc == -1 and b"geometry" in lib.ising_last_error()
     rc = lib.ising_split_pack(1, 1, 1, 4, 100, None)
