@@ -182,6 +182,14 @@ int ising_msc_sync_status(const uint32_t* d_sync, long long rows, int nw, uint32
  * d_scratch2 = ISING_OBS_SCRATCH_BYTES of device memory (any contents). */
 int ising_msc_obs(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, int nw,
                   unsigned long long* d_scratch2, long long* d_out2, void* stream);
+/* Exact (M, E) of a packed lattice like ising_msc_obs, allocating nothing and taking no
+ * scratch: the partial sums go to a fixed 16 KiB device array of the library (one copy per GPU,
+ * kept until the process ends), so any lattice size works. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). Calls that overlap on one GPU race on the array; the array
+ * keeps the last call's partial sums, and every call rewrites the ones it reads. */
+int ising_msc_obs_static(const uint64_t* d_w0, const uint64_t* d_w1, long long rows, int nw,
+                         long long* d_out2, void* stream);
 
 /* ---------------------------------------------------------- batched scans ---- */
 /* No reference counterpart: the reference sweeps one lattice per launch sequence

@@ -64,6 +64,7 @@ SIGNATURES: dict[str, list] = {
     "ising_msc_half_ex": [_P, _P, _LL, _I, _U32, _U32, _U32, _U32, _U32, _U32, _I, _I, _P, _P],
     "ising_msc_sweeps_ex": [_P, _P, _LL, _I, _U32, _LL, _U32, _U32, _U32, _U32, _I, _I, _P, _P],
     "ising_msc_obs": [_P, _P, _LL, _I, _P, _P, _P],
+    "ising_msc_obs_static": [_P, _P, _LL, _I, _P, _P],
     "ising_msc_sweeps_sync": [_P, _P, _LL, _I, _U32, _LL, _U32, _U32, _U32, _U32, _I, _I, _P,
                               _P, _P, _P],
     "ising_msc_sync_bytes": [_LL, _I, _P],

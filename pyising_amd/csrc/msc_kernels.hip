@@ -37,6 +37,7 @@
 #include "msc_core.hpp"
 
 #include <atomic>
+#include <mutex>
 
 // ---------------------------------------------------------------------------- kernels ----
 // stats (optional, diagnostics only): [0] += Philox calls the words needed, [1] += lane-slots
@@ -391,3 +392,43 @@ int ising_msc_obs(const uint64_t* w0, const uint64_t* w1, long long rows, int nw
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------ observables without allocation ----
+// The partial sums of ising_msc_obs_static: one (ones, anti) pair per reduction workgroup, the
+// layout of the caller scratch of ising_msc_obs (ISING_OBS_SCRATCH_BYTES, at most 1024
+// workgroups; any lattice size fits, the reduction loops over grid strides). Each GPU holds its
+// own copy of this array until the process ends. In host code `scratch` names a host-side
+// stand-in, not device memory: host code uses the per-device address from
+// hipGetSymbolAddress (static_scratch) instead, never the name.
+static __device__ __align__(256) unsigned long long scratch[ISING_OBS_SCRATCH_BYTES / 8];
+
+static unsigned long long* g_scratch_addr[64];  // per device: the address of `scratch`
+static std::mutex g_scratch_mu;
+
+// The current device's copy of `scratch`, looked up once per device (with it current).
+static int static_scratch(unsigned long long** out)
+{
+    int dev = 0;
+    ISING_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return ising_fail(ISING_ERANGE, "msc_obs_static: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* ptr = nullptr;
+        ISING_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return ising_fail(ISING_EHIP, "msc_obs_static: no device address for the scratch array");
+        g_scratch_addr[dev] = static_cast<unsigned long long*>(ptr);
+    }
+    *out = g_scratch_addr[dev];
+    return ISING_OK;
+}
+
+extern "C" int ising_msc_obs_static(const uint64_t* w0, const uint64_t* w1, long long rows, int nw,
+                                    long long* out2, void* stream)
+{
+    int rc = msc_check(w0, w1, rows, nw);
+    if (rc) return rc;
+    if (!out2) return ising_fail(ISING_EINVAL, "msc_obs_static: null output");
+    unsigned long long* ptr = nullptr;
+    if ((rc = static_scratch(&ptr))) return rc;
+    return ising_msc_obs(w0, w1, rows, nw, ptr, out2, stream);
+}
