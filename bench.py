@@ -216,17 +216,25 @@ def serving(n_requests: int = 300, rate: float = 200.0, seed: int = 0) -> dict:
     return out
 
 
-def settle_clocks(sim, e0, e1, min_ms: float, chunk: int = 250) -> tuple[int, float]:
-    """Sweep ``sim`` in chunks until at least ``min_ms`` of GPU time has passed (events on the
-    simulation's stream); returns (sweeps run, GPU milliseconds)."""
-    sweeps, ms = 0, 0.0
-    while ms < min_ms:
+def settle_clocks(sim, e0, e1, min_ms: float, max_ms: float = 3000.0, chunk: int = 100,
+                  window: int = 8, tol: float = 0.01) -> tuple[int, float]:
+    """Sweep ``sim`` in chunks until at least ``min_ms`` of GPU time has passed and the last
+    ``window`` chunks agree within ``tol`` (the clocks have stopped moving), or ``max_ms`` has
+    passed (events on the simulation's stream); returns (sweeps run, GPU milliseconds). A GPU
+    that has idled through the set-up takes a few hundred milliseconds of sustained load to
+    reach its steady clock, and loses it again within about 10 ms of idling."""
+    sweeps, ms, recent = 0, 0.0, []
+    while ms < max_ms:
         e0.record(sim.stream_ptr)
         sim.sweep(chunk)
         e1.record(sim.stream_ptr)
         e1.sync()
-        ms += e0.elapsed_ms(e1)
+        t = e0.elapsed_ms(e1)
+        ms += t
         sweeps += chunk
+        recent = (recent + [t])[-window:]
+        if ms >= min_ms and len(recent) == window and max(recent) <= (1 + tol) * min(recent):
+            break
     return sweeps, ms
 
 
@@ -252,8 +260,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--settle-ms", type=float, default=300.0,
-                    help="GPU milliseconds of sweeps before --warmup, to settle the clocks")
+    ap.add_argument("--settle-ms", type=float, default=500.0,
+                    help="GPU milliseconds of sweeps (at least) before --warmup, to settle the "
+                         "clocks; continues until the sweep rate is steady (at most 3 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ceilings", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -291,8 +300,7 @@ def main() -> int:
     # milliseconds of sustained load to reach their steady level after the idle set-up
     settle_sweeps, settle_ms = settle_clocks(sim, e0, e1, args.settle_ms)
     sim.sweep(args.warmup)
-    sim.synchronize()
-    _lib.device_sync()
+    _lib.device_sync()  # (the fused sweeps' status is checked with the observables below)
     t0 = time.perf_counter()
     e0.record(sim.stream_ptr)
     sim.sweep(args.steps)

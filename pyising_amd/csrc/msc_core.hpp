@@ -31,26 +31,32 @@ constexpr u32 b3imm(F f)
     return f(0xF0u, 0xCCu, 0xAAu) & 0xFFu;
 }
 constexpr u32 B3_MAJ = b3imm([](u32 x, u32 y, u32 z) { return (x & y) | (x & z) | (y & z); });
-constexpr u32 B3_OR3 = b3imm([](u32 x, u32 y, u32 z) { return x | y | z; });
-constexpr u32 B3_NOT_A_OR_BC = b3imm([](u32 x, u32 y, u32 z) { return ~(x | (y & z)); });
-constexpr u32 B3_AB_OR_AND_C = b3imm([](u32 x, u32 y, u32 z) { return (x | y) & z; });
-static_assert(B3_MAJ == 0xE8u && B3_OR3 == 0xFEu && B3_NOT_A_OR_BC == 0x07u && B3_AB_OR_AND_C == 0xA8u,
-              "bitop3 truth tables");
+static_assert(B3_MAJ == 0xE8u, "bitop3 truth table");
 
 #define bop3(a, b, c, imm) ((u32)__builtin_amdgcn_bitop3_b32((a), (b), (c), (imm)))
 
-// One 32-bit half: own spins o, other-colour neighbours (c, hz, up, dn). With x_k = o ^ n_k
-// (anti-aligned), a2 = at least two of x1..x4 = maj(x1, x2, x3) | (x4 & (x1 | x2 | x3)),
-// und = ~a2 (lanes that need a comparison), a1 = exactly one = (x1 | x2 | x3 | x4) & und,
-// pw = o ^ a2 (a2 lanes flip unconditionally). Nine VALU ops.
+// One 32-bit half: own spins o, other-colour neighbours n1..n4 = (c, hz, up, dn). With
+// k = the number of neighbours anti-aligned with o: und = k <= 1 (lanes that need a
+// comparison), a1 = k == 1, pw = o ^ (k >= 2) (those lanes flip unconditionally). Counting
+// n1..n3 with a full adder, cy = maj(n1, n2, n3) and sm = n1 ^ n2 ^ n3, k3 <= 1 exactly when
+// cy == o, and k3 == 0 exactly when also sm == o; adding n4: und = (cy == o) & ~((sm ^ o) &
+// (n4 ^ o)) and a1 = und & (sm ^ n4). Six VALU ops (checked against the count over all 32
+// inputs on the host).
+constexpr u32 B3_XOR3 = b3imm([](u32 x, u32 y, u32 z) { return x ^ y ^ z; });
+constexpr u32 B3_BOTH_DIFFER = b3imm([](u32 x, u32 y, u32 z) { return (x ^ z) & (y ^ z); });
+constexpr u32 B3_EQ_AND_NOT = b3imm([](u32 x, u32 y, u32 z) { return ~(x ^ y) & ~z; });
+constexpr u32 B3_AND_XOR = b3imm([](u32 x, u32 y, u32 z) { return x & (y ^ z); });
+static_assert(B3_XOR3 == 0x96u && B3_BOTH_DIFFER == 0x42u && B3_EQ_AND_NOT == 0x41u && B3_AND_XOR == 0x60u,
+              "bitop3 truth tables");
+
 __device__ __forceinline__ void msc_bits(u32 o, u32 c, u32 hz, u32 up, u32 dn, u32& a1, u32& pw,
                                          u32& und)
 {
-    const u32 x1 = o ^ c, x2 = o ^ hz, x3 = o ^ up, x4 = o ^ dn;
-    const u32 m = bop3(x1, x2, x3, B3_MAJ);
-    const u32 r = bop3(x1, x2, x3, B3_OR3);
-    und = bop3(m, x4, r, B3_NOT_A_OR_BC);
-    a1 = bop3(r, x4, und, B3_AB_OR_AND_C);
+    const u32 cy = bop3(c, hz, up, B3_MAJ);
+    const u32 sm = bop3(c, hz, up, B3_XOR3);
+    const u32 g = bop3(sm, dn, o, B3_BOTH_DIFFER);
+    und = bop3(cy, o, g, B3_EQ_AND_NOT);
+    a1 = bop3(und, sm, dn, B3_AND_XOR);
     pw = ~(o ^ und);
 }
 
@@ -502,7 +508,10 @@ __device__ __forceinline__ void msc_phase_a_calls(const MscArgs& a, u32 base, in
     }
 }
 
-template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, class PF, bool PRE = false>
+// FULL: every word of the tile exists (base + 256 R <= n_words), so no word needs the
+// past-the-end guards.
+template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, class PF, bool PRE = false,
+          bool FULL = false>
 __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes, u32 base,
                                                      MscQueueLds<R>& L,
                                                      const MscTileLoads<R, LAY>& T, PF prefetch,
@@ -523,7 +532,7 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
     for (int rr = 0; rr < R; ++rr) {
         const u32 slot = (u32)rr * 256u + tid;
         const u32 w = base + slot;
-        const bool valid = w < a.n_words;
+        const bool valid = FULL || w < a.n_words;
         // the CAP Philox calls depend on the counter only: computed before the spin loads are
         // consumed, they cover the memory latency (at the start of every workgroup above all)
         PhiloxOut r[CAP];

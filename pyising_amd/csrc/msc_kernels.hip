@@ -27,7 +27,7 @@
 //    (0-3 ops per 32-bit half and level) instead of a masked general update;
 //  * loops are wave-uniform (no per-lane exit), so there is no exec-mask bookkeeping; a
 //    decided lane's updates are no-ops;
-//  * classification is nine v_bitop3/XOR ops per 32-bit half; when a wave's 64 words lie in
+//  * classification is six v_bitop3/XOR ops per 32-bit half; when a wave's 64 words lie in
 //    one row (Lx % 8192 == 0) its row pointers are scalar and the horizontal boundary bit
 //    moves between lanes by a DPP wavefront shift;
 //  * queue compaction (mode 0): `cap` Philox calls per word unrolled (computed before the spin
