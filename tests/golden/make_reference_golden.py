@@ -116,6 +116,7 @@ def end_states(cls, random_lattice) -> list[dict]:
 
 
 def main() -> int:
+    sys.dont_write_bytecode = True  # the reference checkout stays untouched (no __pycache__)
     sys.path.insert(0, str(REFERENCE_SRC))
     from pyising.backends.numpy_backend import NumpyMultiSpin
     from pyising.lattice import random_lattice
