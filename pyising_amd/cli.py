@@ -1,108 +1,162 @@
-"""Command line interface: ``python -m pyising_amd run | bench | list | validate``
-(src/pyising/cli.py:1-121: same commands, options and output lines; the backend is ``hip``).
+"""Command line: ``python -m pyising_amd list | run | bench | validate``.
+
+The commands, their options and their output lines are those of the reference's command line
+(src/pyising/cli.py:22-110: ``--backend --variant --dim --L --T --algorithm --seed --init
+--sweeps --warmup``, ``--repeats`` for bench, ``--quick --out`` for validate), with ``hip`` as
+the backend; the reference's ``movie`` command is not provided. Timings bracket the sweeps with
+device synchronisation, as the reference's do.
 """
 
 from __future__ import annotations
 
 import argparse
+import statistics
 import sys
 import time
-
-import numpy as np
+from dataclasses import dataclass
 
 from . import backends as B
 from .lattice import TC_2D, TC_3D
 
+#: options shared by the simulation commands: flag -> (type, default, extra argparse keywords)
+SIM_OPTIONS: dict[str, tuple] = {
+    "--backend": (str, "hip", {"choices": B.BACKENDS}),
+    "--variant": (str, "int8", {}),
+    "--dim": (int, 2, {"choices": (2, 3)}),
+    "--L": (int, 1024, {}),
+    "--T": (float, None, {"help": "temperature (default: the critical temperature of --dim)"}),
+    "--algorithm": (str, "metropolis", {"choices": ("metropolis", "heatbath")}),
+    "--seed": (int, 0, {}),
+    "--init": (str, "hot", {"choices": ("hot", "cold")}),
+    "--sweeps": (int, 100, {}),
+    "--warmup": (int, 10, {}),
+}
 
-def _sim(args) -> B.IsingSimulation:
-    shape = (args.L,) * args.dim
-    T = args.T if args.T is not None else (TC_2D if args.dim == 2 else TC_3D)
-    return B.create(args.backend, args.variant, shape=shape, beta=1.0 / T, seed=args.seed,
-                    algorithm=args.algorithm, init=args.init)
+
+@dataclass(frozen=True)
+class Workload:
+    """One simulation as the options describe it."""
+
+    backend: str
+    variant: str
+    shape: tuple
+    temperature: float
+    seed: int
+    algorithm: str
+    init: str
+
+    @classmethod
+    def from_args(cls, a: argparse.Namespace) -> "Workload":
+        critical = TC_2D if a.dim == 2 else TC_3D
+        return cls(a.backend, a.variant, (a.L,) * a.dim,
+                   critical if a.T is None else a.T, a.seed, a.algorithm, a.init)
+
+    def simulation(self):
+        return B.create(self.backend, self.variant, shape=self.shape, beta=1.0 / self.temperature,
+                        seed=self.seed, algorithm=self.algorithm, init=self.init)
 
 
-def cmd_list(args) -> int:
-    for be, var in B.REGISTRY:
-        cls = B.load(be, var)
-        status = "ok" if cls is not None else "unavailable"
-        extra = (f"dims={cls.dims} algs={cls.algorithms} B/update={cls.bytes_per_update:g}"
-                 if cls else "")
-        print(f"{be:>14} / {var:<18} {status:<12} {extra}")
+def _timed(sim, sweeps: int) -> float:
+    """Seconds for ``sweeps`` sweeps, the device synchronised before and after."""
+    sim.synchronize()
+    start = time.perf_counter()
+    sim.sweep(sweeps)
+    sim.synchronize()
+    return time.perf_counter() - start
+
+
+def _flips_per_ns(sim, sweeps: int, seconds: float) -> float:
+    return sweeps * sim.n_spins / seconds * 1e-9
+
+
+# ------------------------------------------------------------------------- commands ----
+def do_list(a: argparse.Namespace) -> int:
+    """Every registry entry, whether it can run here, and what it supports."""
+    for backend, variant in B.REGISTRY:
+        cls = B.load(backend, variant)
+        if cls is None:
+            status, detail = "unavailable", ""
+        else:
+            status = "ok"
+            detail = (f"dims={cls.dims} algs={cls.algorithms} "
+                      f"B/update={cls.bytes_per_update:g}")
+        print(f"{backend:>14} / {variant:<18} {status:<12} {detail}")
     return 0
 
 
-def cmd_run(args) -> int:
-    sim = _sim(args)
-    sim.sweep(args.warmup)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    sim.sweep(args.sweeps)
-    sim.synchronize()
-    dt = time.perf_counter() - t0
+def do_run(a: argparse.Namespace) -> int:
+    """One timed run: m, e after the sweeps and the rate of this single run."""
+    w = Workload.from_args(a)
+    sim = w.simulation()
+    sim.sweep(a.warmup)
+    seconds = _timed(sim, a.sweeps)
     m, e = sim.observables()
-    rate = args.sweeps * sim.n_spins / dt / 1e9
     print(f"{sim.backend}/{sim.variant} {sim.algorithm} shape={sim.shape} T={1 / sim.beta:.5f}")
-    print(f"m = {m:+.6f}   e = {e:.6f}   {rate:.2f} flips/ns ({dt:.3f} s, single run)")
+    print(f"m = {m:+.6f}   e = {e:.6f}   {_flips_per_ns(sim, a.sweeps, seconds):.2f} flips/ns "
+          f"({seconds:.3f} s, single run)")
     return 0
 
 
-def cmd_bench(args) -> int:
-    """Quick single-configuration timing (python -m pyising_amd.report for a full report)."""
-    sim = _sim(args)
-    sim.sweep(args.warmup)
-    sim.synchronize()
-    rates = []
-    for _ in range(args.repeats):
-        t0 = time.perf_counter()
-        sim.sweep(args.sweeps)
-        sim.synchronize()
-        rates.append(args.sweeps * sim.n_spins / (time.perf_counter() - t0) / 1e9)
-    print(f"{sim.backend}/{sim.variant} L={args.L} dim={args.dim}: median "
-          f"{np.median(rates):.2f} flips/ns (min {min(rates):.2f}, max {max(rates):.2f}); "
-          f"use python -m pyising_amd.report for a full report")
+def do_bench(a: argparse.Namespace) -> int:
+    """Median of --repeats timed runs of --sweeps sweeps (one configuration)."""
+    w = Workload.from_args(a)
+    sim = w.simulation()
+    sim.sweep(a.warmup)
+    rates = [_flips_per_ns(sim, a.sweeps, _timed(sim, a.sweeps)) for _ in range(a.repeats)]
+    print(f"{sim.backend}/{sim.variant} L={a.L} dim={a.dim}: median "
+          f"{statistics.median(rates):.2f} flips/ns (min {min(rates):.2f}, max {max(rates):.2f});"
+          f" use python -m pyising_amd.report for the full size sweep")
     return 0
 
 
-def cmd_validate(args) -> int:
+def do_validate(a: argparse.Namespace) -> int:
+    """The physics validation suite; exit status 1 unless every backend run was identical."""
     from .validate import run
 
-    res = run(quick=args.quick, out=args.out)
-    for k, v in res["summary"].items():
-        print(f"{k:>40}: {v}")
-    return 0 if res["summary"]["all_passed"] else 1
+    summary = run(quick=a.quick, out=a.out)["summary"]
+    width = max(len(k) for k in summary)
+    for key, value in summary.items():
+        print(f"{key:>{max(40, width)}}: {value}")
+    return 0 if summary["all_identical"] else 1
+
+
+#: name -> (help, handler, extra options beyond SIM_OPTIONS, takes SIM_OPTIONS)
+COMMANDS: dict[str, tuple] = {
+    "list": ("list backends/variants and availability", do_list, {}, False),
+    "run": ("run a simulation and print m, e, throughput", do_run, {}, True),
+    "bench": ("quick timing of one configuration", do_bench,
+              {"--repeats": (int, 5, {})}, True),
+    "validate": ("physics validation suite", do_validate,
+                 {"--quick": (None, False, {"action": "store_true"}),
+                  "--out": (str, "results/validation.json", {})}, False),
+}
+
+
+def _add(sp: argparse.ArgumentParser, flag: str, spec: tuple) -> None:
+    typ, default, extra = spec
+    kw = dict(extra)
+    if typ is not None:
+        kw["type"] = typ
+    sp.add_argument(flag, default=default, **kw)
 
 
 def parser() -> argparse.ArgumentParser:
-    p = argparse.ArgumentParser(prog="python -m pyising_amd", description=__doc__)
+    p = argparse.ArgumentParser(prog="pyising_amd", description=__doc__,
+                                formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = p.add_subparsers(dest="cmd", required=True)
-
-    def common(sp, sweeps: int = 100) -> None:
-        sp.add_argument("--backend", default="hip", choices=B.BACKENDS)
-        sp.add_argument("--variant", default="int8")
-        sp.add_argument("--dim", type=int, default=2, choices=(2, 3))
-        sp.add_argument("--L", type=int, default=1024)
-        sp.add_argument("--T", type=float, default=None, help="temperature (default T_c)")
-        sp.add_argument("--algorithm", default="metropolis", choices=("metropolis", "heatbath"))
-        sp.add_argument("--seed", type=int, default=0)
-        sp.add_argument("--init", default="hot", choices=("hot", "cold"))
-        sp.add_argument("--sweeps", type=int, default=sweeps)
-        sp.add_argument("--warmup", type=int, default=10)
-
-    sub.add_parser("list", help="list backends/variants and availability")
-    common(sub.add_parser("run", help="run a simulation and print m, e, throughput"))
-    sb = sub.add_parser("bench", help="quick timing of one configuration")
-    common(sb)
-    sb.add_argument("--repeats", type=int, default=5)
-    sv = sub.add_parser("validate", help="physics validation suite")
-    sv.add_argument("--quick", action="store_true")
-    sv.add_argument("--out", default="results/validation.json")
+    for name, (help_text, handler, extra, with_sim) in COMMANDS.items():
+        sp = sub.add_parser(name, help=help_text)
+        sp.set_defaults(handler=handler)
+        for flag, spec in (SIM_OPTIONS.items() if with_sim else ()):
+            _add(sp, flag, spec)
+        for flag, spec in extra.items():
+            _add(sp, flag, spec)
     return p
 
 
 def main(argv: list[str] | None = None) -> int:
     args = parser().parse_args(argv)
-    return {"list": cmd_list, "run": cmd_run, "bench": cmd_bench,
-            "validate": cmd_validate}[args.cmd](args)
+    return args.handler(args)
 
 
 if __name__ == "__main__":
