@@ -16,12 +16,12 @@ from ..lattice import check_shape, threshold_table
 
 
 class IsingSimulation(ABC):
-    """A checkerboard Ising Monte Carlo simulation living on one backend.
+    """One lattice and its sweep counter on one backend.
 
-    shape: ``(L, L)`` or ``(L, L, L)``; extents even, last extent a multiple of
-    :attr:`layout_multiple`. beta: inverse temperature. seed: 64-bit Philox key.
-    algorithm: ``"metropolis"`` or ``"heatbath"``. init: ``"hot"``, ``"cold"`` or an explicit
-    int8 lattice.
+    The constructor takes the lattice extents (2D or 3D, each even, the last one a multiple of
+    :attr:`layout_multiple`), the inverse temperature, the 64-bit key of the random stream, the
+    update rule (Metropolis or heat bath) and the start: random (``"hot"``), all +1
+    (``"cold"``) or a given int8 lattice of +1 / -1.
     """
 
     backend: ClassVar[str] = "base"
@@ -29,9 +29,11 @@ class IsingSimulation(ABC):
     dims: ClassVar[tuple[int, ...]] = (2, 3)
     algorithms: ClassVar[tuple[str, ...]] = ("metropolis", "heatbath")
     layout_multiple: ClassVar[int] = 8
-    #: random-stream family: same stream, seed and start give bit-identical lattices
+    #: which random stream the variant draws from; two simulations drawing from the same
+    #: stream with the same key and start stay bit-identical
     stream: ClassVar[str] = "int8"
-    #: minimal DRAM traffic per attempted spin update (read own + neighbours, write own)
+    #: the traffic model: bytes moved per attempted update (own word read and written, the
+    #: other colour read once)
     bytes_per_update: ClassVar[float] = 3.0
     gpu: ClassVar[bool] = True
 
@@ -65,13 +67,13 @@ class IsingSimulation(ABC):
         return int(np.prod(self.shape))
 
     def set_beta(self, beta: float) -> None:
-        """Change the temperature; keeps the configuration and the sweep counter."""
+        """New inverse temperature from the next sweep on; spins and sweep counter stay."""
         self.beta = float(beta)
         self.table = threshold_table(self.beta, self.ndim, self.algorithm)
         self._upload_table()
 
     def sweep(self, n: int = 1) -> None:
-        """Run ``n`` full checkerboard sweeps (colour 0 then colour 1)."""
+        """``n`` sweeps; each updates colour 0, then colour 1, then advances the counter."""
         for _ in range(int(n)):
             self._half_sweep(0)
             self._half_sweep(1)
@@ -79,22 +81,23 @@ class IsingSimulation(ABC):
 
     @abstractmethod
     def _setup(self) -> None:
-        """Prepare device state (the spins are allocated by the init path)."""
+        """Allocate what the backend keeps besides the spins, which the start fills."""
 
     @abstractmethod
     def _half_sweep(self, color: int) -> None:
-        """Update every site of one colour using sweep counter ``self.t``."""
+        """All sites of one colour, drawing from the stream at sweep counter ``self.t``."""
 
     @abstractmethod
     def get_lattice(self) -> np.ndarray:
-        """The configuration as a full int8 host array."""
+        """A host copy of the lattice, int8 +1 / -1, in the constructor's shape."""
 
     @abstractmethod
     def set_lattice(self, lat: np.ndarray) -> None:
-        """Load a full int8 host configuration."""
+        """Replace the spins by a host lattice of the constructor's shape."""
 
     def _upload_table(self) -> None:  # noqa: B027 - optional hook
-        """Push ``self.table`` to the device (tables travel as kernel arguments here)."""
+        """Hand a new threshold table to the device (the hip kernels take it as arguments,
+        so nothing is uploaded)."""
 
     @abstractmethod
     def _fill_hot(self) -> None:
@@ -104,7 +107,7 @@ class IsingSimulation(ABC):
         self.set_lattice(np.ones(self.shape, dtype=np.int8))
 
     def synchronize(self) -> None:  # noqa: B027 - optional hook
-        """Block until all queued device work is done."""
+        """Return once the device has finished the simulation's queued work."""
 
     @abstractmethod
     def observables_exact(self) -> tuple[int, int]:
@@ -122,5 +125,5 @@ class IsingSimulation(ABC):
         return self.observables()[1]
 
     def device_bytes(self) -> int:
-        """Bytes of lattice state held on the device."""
+        """Device bytes the spins occupy."""
         return int(self.n_spins * self.bytes_per_update / 3.0)
