@@ -196,9 +196,9 @@ int ising_msc_obs_static(const uint64_t* d_w0, const uint64_t* d_w1, long long r
  * (src/pyising/backends/base.py:85-90). A batch is a list of lattices (int8 2D/3D either
  * algorithm, or msc 2D), each with its own shape, thresholds, seed and sweep counter, whose
  * device colour arrays stay owned by the caller. Every half sweep of all of them is one launch
- * (default), or a whole call is one persistent cooperative launch with a grid barrier between
- * half sweeps (ising_batch_set_persistent); every lattice's result is bit-identical to sweeping
- * it alone. After every sweep M = sum of spins is accumulated per lattice: sum M, sum |M|,
+ * (default), or up to 512 sweeps are one dependency-ordered launch in which each tile waits only
+ * for the tiles of its own lattice that hold its neighbour rows in the previous half sweep
+ * (ising_batch_set_persistent); every lattice's result is bit-identical to sweeping it alone. After every sweep M = sum of spins is accumulated per lattice: sum M, sum |M|,
  * sum M^2 and sum M^4 (128-bit) for Binder cumulants; a batched lattice holds at most 2^28
  * spins (ISING_ERANGE beyond), so the sums stay exact for 65536 sweeps
  * (src/pyising/observables.py:39-42). */
@@ -223,14 +223,15 @@ typedef struct {
                           * the others go on (continuous batching: no step is cut short to
                           * the lattice that needs the fewest sweeps) */
 } IsingBatchLattice;
-/* blocks_per_cu <= 0: occupancy maximum; msc_mode 0 = queue-compaction tiles (512 words),
- * 1 = lockstep tiles (256 words); msc_cap = unrolled calls of the queue tiles (must be 4). */
+/* blocks_per_cu: accepted and unused (launch sizes follow the tile counts); msc_mode 0 =
+ * queue-compaction tiles (512 words), 1 = lockstep tiles (256 words); msc_cap = unrolled calls
+ * of the queue tiles (must be 4). */
 int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, int msc_mode,
                        int msc_cap, void** handle);
-/* n_sweeps sweeps of every lattice (asynchronous; launch form or persistent form). */
+/* n_sweeps sweeps of every lattice (asynchronous; launch form or dependency-ordered form). */
 int ising_batch_sweeps(void* handle, long long n_sweeps, void* stream);
-/* Capture one launch of sweeps_per_replay sweeps into a hipGraph on `stream` (a created
- * stream); ISING_EHIP if the runtime cannot capture it. Replays continue the sweep counter. */
+/* Capture the launches of sweeps_per_replay sweeps into a hipGraph on `stream` (a created
+ * stream); ISING_EHIP if the runtime cannot capture them. Replays continue the sweep counter. */
 int ising_batch_graph(void* handle, long long sweeps_per_replay, void* stream);
 int ising_batch_replay(void* handle, int replays, void* stream);
 /* Synchronous read of n x 7 int64 accumulators (sum M, sum |M|, sum M^2 lo, hi, sum M^4 lo,
@@ -239,9 +240,16 @@ int ising_batch_stats(void* handle, long long* out, long long* sweeps_done, void
 int ising_batch_reset_stats(void* handle, void* stream);
 int ising_batch_info(void* handle, int* grid, int* total_tiles, int* max_blocks_per_cu);
 /* Form of a sweep call: 0 (default) = one launch per half sweep (+ one accumulation launch per
- * sweep while moments are on); 1 = one persistent cooperative launch with a grid barrier
- * between half sweeps. */
+ * sweep while moments are on); 1 = dependency-ordered launches of up to 512 sweeps each (per-tile
+ * completion counters; each lattice runs on one XCD, or in eight bands, one per XCD), followed
+ * by a one-workgroup check of the counters. 1 needs a batch made by ising_batch_create and a
+ * device that places workgroups round-robin over its XCDs or all on one (ISING_EINVAL
+ * otherwise). */
 int ising_batch_set_persistent(void* handle, int persistent);
+/* Failure bits of the dependency-ordered launches since create (synchronizes `stream`): 1 = a
+ * tile's bounded wait gave up, 2 = a tile did not publish its last half sweep. After a failure
+ * the lattices and sums are undefined. */
+int ising_batch_status(void* handle, int* failed, void* stream);
 /* 1 (default): accumulate the moments of M after every sweep (one more launch per sweep in the
  * launch form); 0: skip it, e.g. in a serving loop that reads each lattice's own (M, E). */
 int ising_batch_set_moments(void* handle, int on);

@@ -77,6 +77,7 @@ SIGNATURES: dict[str, list] = {
     "ising_batch_reset_stats": [_P, _P],
     "ising_batch_info": [_P, _P, _P, _P],
     "ising_batch_set_persistent": [_P, _I],
+    "ising_batch_status": [_P, _P, _P],
     "ising_batch_set_moments": [_P, _I],
     "ising_batch_set_tiles_per_workgroup": [_P, _I],
     "ising_batch_destroy": [_P],

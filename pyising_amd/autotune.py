@@ -4,11 +4,12 @@ A batch's speed depends on choices that never change its results: the tile of ea
 lattice size (queue compaction over 512 words, or lockstep over 256 words; small lattices
 have too few words to fill waves after compaction), the sites per lane of each int8 lattice
 size (16, 8 or 4: wide vectors amortise per-lane work, narrow ones fill the lanes of a small
-lattice's tile), the launch form (one launch per half sweep, or one persistent cooperative
-launch), and the number of sweeps per captured graph. :func:`tune_batch` times candidates on
-scratch lattices of the same shapes, temperatures and algorithms (hot starts from other seeds,
-so the callers' lattices are untouched), by coordinate descent: the msc tile per shape, the
-int8 width per shape, then the launch form, then the graph length.
+lattice's tile), the launch form (one launch per half sweep, or dependency-ordered launches of many sweeps)
+together with the tiles each workgroup runs, and the number of sweeps per captured graph.
+:func:`tune_batch` times candidates on scratch lattices of the same shapes, temperatures and
+algorithms (hot starts from other seeds, so the callers' lattices are untouched), by
+coordinate descent: the msc tile per shape, the int8 width per shape, then the launch form and
+tiles per workgroup, then the graph length.
 Results are cached per batch signature in a JSON file outside the repository
 (``$PYISING_AMD_TUNE_CACHE`` or ``~/.cache/pyising_amd/batch_tune.json``).
 """
@@ -21,7 +22,7 @@ from dataclasses import asdict, dataclass, field
 from pathlib import Path
 
 GRAPH_CHOICES = (0, 10, 50)
-TILES_PER_WG_CHOICES = (1, 2, 4)  # launch form: tiles (lattices or parts of one) per workgroup
+TILES_PER_WG_CHOICES = (1, 2, 4)  # tiles (lattices or parts of one) per workgroup, either form
 
 
 @dataclass
@@ -176,11 +177,11 @@ def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
             vecs = dict(best.int8_vecs)
             vecs[sh] = v
             consider(copy(int8_vecs=vecs))
-    consider(copy(persistent=not best.persistent))
-    if not best.persistent:
+    start = (best.persistent, best.tiles_per_workgroup)
+    for persistent in (False, True):
         for k in TILES_PER_WG_CHOICES:
-            if k != best.tiles_per_workgroup:
-                consider(copy(tiles_per_workgroup=k))
+            if (persistent, k) != start:
+                consider(copy(persistent=persistent, tiles_per_workgroup=k))
     for g in GRAPH_CHOICES:
         if g != best.graph_sweeps:
             consider(copy(graph_sweeps=g))

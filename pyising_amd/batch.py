@@ -1,5 +1,6 @@
 """Batched temperature scans: many lattices of mixed size, dimension, variant and temperature
-swept by one persistent GPU kernel (``csrc/batch_kernels.hip``).
+swept together on the GPU (``csrc/batch_kernels.hip``): one launch per half sweep, or
+dependency-ordered launches of many sweeps.
 
 No counterpart in the reference, which sweeps one lattice per launch sequence
 (src/pyising/backends/base.py:85-90). Every lattice keeps its own seed and sweep counter, so a
@@ -104,6 +105,7 @@ class _Members:
     which the simulation's stream would otherwise not wait for."""
 
     _done = None
+    _unchecked = False  # dependency-ordered launches queued since the last status check
     sims: list = []
 
     def _mark(self) -> None:
@@ -113,7 +115,10 @@ class _Members:
 
     def order_after(self, stream_ptr) -> None:
         """Work queued on ``stream_ptr`` from now on waits for the batch's last recorded
-        sweep (on the device; the host does not wait)."""
+        sweep (on the device; the host does not wait, except to check dependency-ordered
+        launches that have not been checked yet)."""
+        if self._unchecked:
+            self.check()
         if self._done is not None:
             _lib.check(self._L.ising_stream_wait_event(stream_ptr, self._done.ptr),
                        "stream_wait_event")
@@ -139,13 +144,14 @@ class _Members:
 
 
 class BatchScan(_Members):
-    """Sweep a list of hip simulations together with one persistent kernel per call.
+    """Sweep a list of hip simulations together, every lattice in the same launches.
 
-    ``blocks_per_cu``: persistent grid size per CU (None = the occupancy maximum).
+    ``blocks_per_cu``: accepted for compatibility; the launch sizes follow the tile counts.
     ``msc_mode``: 0 = queue compaction (512-word tiles), 1 = lockstep tiles.
-    ``persistent``: True = one cooperative launch per ``sweep`` call with a grid barrier between
-    half sweeps; False (default) = one launch per half sweep over all lattices plus one
-    accumulation launch per sweep (kernel boundaries are cheaper than a grid barrier here).
+    ``persistent``: True = dependency-ordered launches (up to 512 sweeps per launch: each tile
+    waits only for the tiles of its own lattice holding its neighbour rows in the previous half
+    sweep, through per-tile completion counters); False (default) = one launch per half sweep
+    over all lattices plus one accumulation launch per sweep.
     The simulations keep ownership of their device arrays; while they belong to the batch (until
     ``close``) their reads and writes (``get_lattice``, ``observables``, ``set_lattice``) are
     queued behind the batch's last sweep, their own ``sweep`` is refused, and their ``t``
@@ -235,13 +241,30 @@ class BatchScan(_Members):
                 self._left[i] = left - step
 
     def sweep(self, n: int = 1) -> None:
-        """n sweeps of every lattice in one persistent launch (asynchronous on the batch stream)."""
+        """n sweeps of every lattice (asynchronous on the batch stream)."""
         n = int(n)
         if n <= 0:
             return
         _lib.check(self._L.ising_batch_sweeps(self._h, n, self.stream.ptr), "batch_sweeps")
+        self._unchecked = self.persistent
         self._mark()
         self._advance(n)
+
+    def _status(self) -> int:
+        failed = C.c_int()
+        _lib.check(self._L.ising_batch_status(self._h, C.byref(failed), self.stream.ptr),
+                   "batch_status")
+        self._unchecked = False
+        return failed.value
+
+    def check(self) -> None:
+        """Raise if a dependency-ordered launch failed (a bounded wait gave up, or a tile did
+        not publish its last half sweep); the lattices and moments are then invalid.
+        Synchronous."""
+        failed = self._status()
+        if failed:
+            raise _lib.HipError(f"dependency-ordered batch launch failed (status {failed}): "
+                                "the lattices are invalid; use persistent=False")
 
     def capture(self, sweeps_per_replay: int) -> bool:
         """Capture one launch into a hipGraph; False if the runtime cannot capture it."""
@@ -257,13 +280,18 @@ class BatchScan(_Members):
             raise RuntimeError("no captured graph: call capture() first")
         _lib.check(self._L.ising_batch_replay(self._h, int(replays), self.stream.ptr),
                    "batch_replay")
+        self._unchecked = self.persistent
         self._mark()
         self._advance(self.graph_sweeps * int(replays))
 
     def synchronize(self) -> None:
         self.stream.sync()
+        if self._unchecked:
+            self.check()
 
     def moments(self) -> list[Moments]:
+        if self._unchecked:
+            self.check()
         out = np.zeros((len(self.sims), 7), dtype=np.int64)
         done = C.c_longlong()
         _lib.check(self._L.ising_batch_stats(self._h, out.ctypes.data, C.byref(done),
@@ -286,11 +314,14 @@ class BatchScan(_Members):
         the member simulations leave the batch and may sweep on their own again."""
         if getattr(self, "_h", None):
             self.stream.sync()
+            failed = self._status() if self._unchecked else 0
             self._L.ising_batch_destroy(self._h)
             self._h = None
             self._release_members()
             if getattr(self, "_own_stream", False):
                 self.stream.close()
+            if failed:
+                raise _lib.HipError(f"dependency-ordered batch launch failed (status {failed})")
 
     def __del__(self):
         if not _lib.releasing_allowed():

@@ -1,5 +1,5 @@
-// Batched temperature scans: one launch per half sweep (or one persistent launch per call)
-// sweeps many lattices of uneven size, dimension, variant and temperature (no reference
+// Batched temperature scans: one launch per half sweep (or one dependency-ordered launch per
+// call) sweeps many lattices of uneven size, dimension, variant and temperature (no reference
 // counterpart; the reference runs one lattice per launch sequence,
 // src/pyising/backends/base.py:85-90).
 //
@@ -10,24 +10,25 @@
 // functions, same counters: t = t0 + sweep), so a batched lattice equals the same lattice swept
 // alone; a lattice with a sweep limit sits the batch's later sweeps out.
 //
-// Two forms give identical results. Default: one launch per half sweep over all lattices (the
-// kernel boundary orders the colours), workgroups taking 1, 2, 4 or 8 tiles of a launch order
-// that balances every lattice over the eight XCDs, long tiles first. Persistent: one
-// cooperative launch per call whose grid strides over a tile map, with a grid barrier between
-// half sweeps. On MI355X the kernel boundary is the cheaper barrier: a grid barrier is a
-// contended cross-XCD atomic on one address per workgroup.
+// Two forms give identical results. Launch form: one launch per half sweep over all lattices
+// (the kernel boundary orders the colours), workgroups taking 1, 2, 4 or 8 tiles of a launch
+// order that balances every lattice over the eight XCDs, long tiles first. Dependency-ordered
+// form: one launch holds the tiles of many half sweeps, and a tile of half sweep h waits only
+// for the tiles of its own lattice that hold its neighbour rows in half sweep h - 1 (per-tile
+// completion counters) instead of for the whole grid: the half sweeps of different lattices,
+// and of distant parts of one lattice, overlap, and no launch boundary or grid barrier drains
+// the GPU between half sweeps.
 //
-// Binder accumulation. While writing a colour, each tile stores its number of +1 spins in its
-// own slot (slots double-buffered by sweep parity; no atomics, so no contention on a
-// per-lattice counter). After a sweep, one workgroup (launch form: in the next sweep's colour-0
-// launch or, after a call's last sweep, its own launch) or one wave (persistent form) per
-// lattice sums its slots, forms
+// Binder accumulation. Launch form: while writing a colour, each tile stores its number of +1
+// spins in its own slot (slots double-buffered by sweep parity; no atomics, so no contention on
+// a per-lattice counter). After a sweep, one workgroup per lattice (in the next sweep's colour-0
+// launch or, after a call's last sweep, its own launch) sums its slots, forms
 // M = 2 * plus - N and accumulates sum M, sum |M| (64-bit: |M| <= N) and sum M^2, sum M^4 as
-// 128-bit (lo, hi) pairs. The number of sweeps
-// done lives in device memory, so a captured graph can be replayed and every replay continues
-// the counter stream.
-#include <hip/hip_cooperative_groups.h>
-
+// 128-bit (lo, hi) pairs. Dependency-ordered form: each tile stores its count in its slot, and
+// a fold entry per lattice at the end of every colour-1 half sweep sums the slots once the
+// lattice's tiles are done and folds M into the sums with atomics. The number of sweeps done
+// lives in device memory, so a captured graph can be replayed and every replay continues the
+// counter stream.
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
@@ -36,8 +37,7 @@
 
 #include "int8_core.hpp"
 #include "msc_core.hpp"
-
-namespace cg = cooperative_groups;
+#include "placement.hpp"
 
 // The C ABI struct's layout, which the Python mirror (pyising_amd/batch.py) repeats.
 static_assert(sizeof(IsingBatchLattice) == 96 && offsetof(IsingBatchLattice, t0) == 80 &&
@@ -64,6 +64,9 @@ struct BatchDesc {
     u32 k0, k1, t0;
     u32 limit;  // batch sweeps (since create / assign) this lattice takes part in
     u32 tile_begin, n_tiles;
+    // dependency-ordered form: tile i of a half sweep needs tiles i - dep_span .. i + dep_span
+    // (cyclic) of the previous one, or every tile when dep_all
+    u32 dep_span, dep_all;
     long long n_spins;
 };
 
@@ -136,42 +139,32 @@ __device__ __forceinline__ void batch_fold_moments(BatchAcc& A, const BatchDesc&
     A.samples += 1;
 }
 
-__device__ __forceinline__ void batch_accumulate_lattice(const BatchKernelArgs& ka, u32 lat, u32 par,
-                                                         u32 sw)
-{
-    const BatchDesc& d = ka.descs[lat];
-    if (sw >= d.limit) return;  // the lattice sat this sweep out (wave-uniform)
-    u64 plus = batch_lattice_plus(ka, d, par, threadIdx.x & 63u, 64u);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) plus += __shfl_xor(plus, off, 64);
-    if ((threadIdx.x & 63) == 0) batch_fold_moments(ka.acc[lat], d, plus);
-}
-
-
 // One 512-word msc tile by queue compaction (4 unrolled calls, then packed rounds), with the
-// lattice's wave classification (block-uniform branch).
+// lattice's wave classification (block-uniform branch). COH: loads that miss in the L1.
+template <bool COH>
 __device__ __forceinline__ u32 batch_msc_tile(const MscArgs& a, u64 codes, u32 base, u32 layout,
                                               MscQueueLds<2>& lds)
 {
     u32 calls = 0, slots = 0, words = 0;
     if (layout == MSC_ROWW)
-        return msc_queue_tile<2, 4, MSC_ROWW, false, true, true>(a, codes, base, lds, calls, slots, words);
+        return msc_queue_tile<2, 4, MSC_ROWW, false, true, true, COH>(a, codes, base, lds, calls, slots, words);
     if (layout == MSC_SEGW)
-        return msc_queue_tile<2, 4, MSC_SEGW, false, true, true>(a, codes, base, lds, calls, slots, words);
-    return msc_queue_tile<2, 4, MSC_GENERAL, false, true, true>(a, codes, base, lds, calls, slots, words);
+        return msc_queue_tile<2, 4, MSC_SEGW, false, true, true, COH>(a, codes, base, lds, calls, slots, words);
+    return msc_queue_tile<2, 4, MSC_GENERAL, false, true, true, COH>(a, codes, base, lds, calls, slots, words);
 }
 
 // One 256-word lockstep msc tile; returns the +1 spins written. With a whole-wave layout the
 // word count is a multiple of 64, so the validity test is wave-uniform.
+template <bool COH>
 __device__ __forceinline__ u32 batch_msc_lockstep(const MscArgs& a, u64 codes, u32 local,
                                                   u32 layout)
 {
     const u32 w = local * 256u + threadIdx.x;
     MscState st = {};
     if (w < a.n_words) {
-        if (layout == MSC_ROWW) st = msc_classify_wave<MSC_ROWW>(a, w);
-        else if (layout == MSC_SEGW) st = msc_classify_wave<MSC_SEGW>(a, w);
-        else st = msc_classify(a, w);
+        if (layout == MSC_ROWW) st = msc_classify_wave<MSC_ROWW, COH>(a, w);
+        else if (layout == MSC_SEGW) st = msc_classify_wave<MSC_SEGW, COH>(a, w);
+        else st = msc_classify_wave<MSC_GENERAL, COH>(a, w);
     }
     u32 calls = 0, slots = 0;
     msc_run<true, false>(a, codes, w, 0u, 16u, st, calls, slots);
@@ -183,8 +176,8 @@ __device__ __forceinline__ u32 batch_msc_lockstep(const MscArgs& a, u64 codes, u
 }
 
 // One int8 tile; thr_lds is this workgroup's threshold table for the lattice (filled by
-// batch_int8 below, which all threads call).
-template <int NDIM, int HB, int V>
+// batch_int8_v below, which all threads call).
+template <int NDIM, int HB, int V, bool COH>
 __device__ __forceinline__ u32 batch_int8_tile(const BatchDesc& d, const u32* thr_lds, u32 local,
                                                u32 t, u32 color)
 {
@@ -192,76 +185,55 @@ __device__ __forceinline__ u32 batch_int8_tile(const BatchDesc& d, const u32* th
     if (ci >= d.g.n_chunks) return 0u;
     int8_t* own = (int8_t*)(color ? d.c1 : d.c0);
     const int8_t* other = (const int8_t*)(color ? d.c0 : d.c1);
-    const u32 minus = int8_update_chunk<NDIM, HB, V, true>(own, other, d.g, thr_lds, t, color,
-                                                            d.k0, d.k1, ci);
+    const u32 minus = int8_update_chunk<NDIM, HB, V, true, COH>(own, other, d.g, thr_lds, t, color,
+                                                                 d.k0, d.k1, ci);
     return (u32)V - minus;
 }
 
-// REUSE: the workgroup ran an earlier tile whose threads may still read thr_lds.
-template <int NDIM, int HB, bool REUSE>
+template <int NDIM, int HB, bool COH>
 __device__ __forceinline__ u32 batch_int8_v(const BatchDesc& d, u32* thr_lds, u32 local, u32 t,
                                             u32 color)
 {
-    if (REUSE) __syncthreads();
     int8_fill_table<NDIM, HB>(thr_lds, d.tb);
     __syncthreads();
-    if (d.vec == 16) return batch_int8_tile<NDIM, HB, 16>(d, thr_lds, local, t, color);
-    if (d.vec == 8) return batch_int8_tile<NDIM, HB, 8>(d, thr_lds, local, t, color);
-    return batch_int8_tile<NDIM, HB, 4>(d, thr_lds, local, t, color);
+    if (d.vec == 16) return batch_int8_tile<NDIM, HB, 16, COH>(d, thr_lds, local, t, color);
+    if (d.vec == 8) return batch_int8_tile<NDIM, HB, 8, COH>(d, thr_lds, local, t, color);
+    return batch_int8_tile<NDIM, HB, 4, COH>(d, thr_lds, local, t, color);
 }
 
-__global__ __launch_bounds__(256) void batch_persistent(BatchKernelArgs ka)
+// Tile kinds a kernel instantiates: every kind, every kind but the msc queue-compaction tiles
+// (whose phase A holds 32 more registers), or only those.
+enum : int { TILES_ALL = 0, TILES_LIGHT = 1, TILES_QUEUE = 2 };
+
+// Tile `local` of lattice d, colour `color`, Philox sweep counter t: this thread's count of +1
+// spins written. Every thread of the workgroup calls it; callers that ran an earlier tile put a
+// barrier first (its threads may still read thr_lds). COH: lattice loads miss in the L1. SET:
+// the kinds compiled in (the caller only passes lattices of those kinds).
+template <bool COH, int SET = TILES_ALL>
+__device__ __forceinline__ u32 batch_tile(const BatchDesc& d, MscQueueLds<2>& lds, u32* thr_lds,
+                                          u32 local, u32 t, u32 color)
 {
-    __shared__ MscQueueLds<2> lds;
-    __shared__ u32 thr_lds[8];
-    cg::grid_group grid = cg::this_grid();
-    const u32 done0 = (u32)*ka.sweeps_done;
-    const u32 gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const u32 gthreads = gridDim.x * blockDim.x;
-    for (u32 s = 0; s < ka.n_sweeps; ++s) {
-        const u32 sw = done0 + s;
-        const u32 par = sw & 1u;
-        for (u32 color = 0; color < 2; ++color) {
-            for (u32 tile = blockIdx.x; tile < ka.total_tiles; tile += gridDim.x) {
-                const u32 lat = ka.tile_lat[tile];
-                const BatchDesc& d = ka.descs[lat];
-                if (sw >= d.limit) continue;  // past the lattice's sweep limit (block-uniform)
-                const u32 local = tile - d.tile_begin;
-                const u32 t = d.t0 + sw;
-                u32 plus = 0;
-                if (d.kind == BATCH_INT8) {
-                    if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1, true>(d, thr_lds, local, t, color)
-                                                 : batch_int8_v<2, 0, true>(d, thr_lds, local, t, color);
-                    else plus = d.hb ? batch_int8_v<3, 1, true>(d, thr_lds, local, t, color)
-                                     : batch_int8_v<3, 0, true>(d, thr_lds, local, t, color);
-                } else {
-                    MscArgs a;
-                    a.own = (u64*)(color ? d.c1 : d.c0);
-                    a.other = (const u64*)(color ? d.c0 : d.c1);
-                    a.rows = d.rows;
-                    a.nw = d.nw;
-                    a.n_words = d.n_words;
-                    a.div_nw = d.div_nw;
-                    a.t = t;
-                    a.color = color;
-                    a.k0 = d.k0;
-                    a.k1 = d.k1;
-                    plus = d.msc_queue ? batch_msc_tile(a, d.codes, local * 512u, d.layout, lds)
-                                       : batch_msc_lockstep(a, d.codes, local, d.layout);
-                }
-                wave_store_sum(ka.plus + ((size_t)par * 2 + color) * BATCH_WAVES * ka.total_tiles +
-                                   (size_t)BATCH_WAVES * tile, plus);
-            }
-            grid.sync();
-        }
-        // M of every lattice, one wave each; this parity's slots are next written two grid
-        // barriers later, so the next sweep can start right away.
-        const u32 gwave = gtid >> 6, gwaves = gthreads >> 6;
-        for (u32 lat = gwave; lat < ka.n_lat; lat += gwaves)
-            batch_accumulate_lattice(ka, lat, par, sw);
+    if (SET != TILES_QUEUE && d.kind == BATCH_INT8) {
+        if (d.ndim == 2) return d.hb ? batch_int8_v<2, 1, COH>(d, thr_lds, local, t, color)
+                                     : batch_int8_v<2, 0, COH>(d, thr_lds, local, t, color);
+        return d.hb ? batch_int8_v<3, 1, COH>(d, thr_lds, local, t, color)
+                    : batch_int8_v<3, 0, COH>(d, thr_lds, local, t, color);
     }
-    grid.sync();
-    if (gtid == 0) *ka.sweeps_done = done0 + ka.n_sweeps;
+    MscArgs a;
+    a.own = (u64*)(color ? d.c1 : d.c0);
+    a.other = (const u64*)(color ? d.c0 : d.c1);
+    a.rows = d.rows;
+    a.nw = d.nw;
+    a.n_words = d.n_words;
+    a.div_nw = d.div_nw;
+    a.t = t;
+    a.color = color;
+    a.k0 = d.k0;
+    a.k1 = d.k1;
+    if constexpr (SET == TILES_QUEUE) return batch_msc_tile<COH>(a, d.codes, local * 512u, d.layout, lds);
+    else if constexpr (SET == TILES_LIGHT) return batch_msc_lockstep<COH>(a, d.codes, local, d.layout);
+    else return d.msc_queue ? batch_msc_tile<COH>(a, d.codes, local * 512u, d.layout, lds)
+                            : batch_msc_lockstep<COH>(a, d.codes, local, d.layout);
 }
 
 // ------------------------------------------------------------- launch-per-half-sweep form ----
@@ -302,28 +274,7 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color,
         if (sw >= d.limit) continue;  // past the lattice's sweep limit: the whole workgroup idles
         if (j) __syncthreads();  // an earlier tile of this workgroup may still read thr_lds
         const u32 local = tile - d.tile_begin;
-        const u32 t = d.t0 + sw;
-        u32 plus = 0;
-        if (d.kind == BATCH_INT8) {
-            if (d.ndim == 2) plus = d.hb ? batch_int8_v<2, 1, false>(d, thr_lds, local, t, color)
-                                         : batch_int8_v<2, 0, false>(d, thr_lds, local, t, color);
-            else plus = d.hb ? batch_int8_v<3, 1, false>(d, thr_lds, local, t, color)
-                             : batch_int8_v<3, 0, false>(d, thr_lds, local, t, color);
-        } else {
-            MscArgs a;
-            a.own = (u64*)(color ? d.c1 : d.c0);
-            a.other = (const u64*)(color ? d.c0 : d.c1);
-            a.rows = d.rows;
-            a.nw = d.nw;
-            a.n_words = d.n_words;
-            a.div_nw = d.div_nw;
-            a.t = t;
-            a.color = color;
-            a.k0 = d.k0;
-            a.k1 = d.k1;
-            plus = d.msc_queue ? batch_msc_tile(a, d.codes, local * 512u, d.layout, lds)
-                               : batch_msc_lockstep(a, d.codes, local, d.layout);
-        }
+        const u32 plus = batch_tile<false>(d, lds, thr_lds, local, d.t0 + sw, color);
         if (ka.moments)
             wave_store_sum(ka.plus + ((size_t)par * 2 + color) * BATCH_WAVES * ka.total_tiles +
                                (size_t)BATCH_WAVES * tile, plus);
@@ -347,6 +298,263 @@ __global__ void batch_zero_u64(unsigned long long* p, u32 n)
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
+// ------------------------------------------------------------- dependency-ordered form ----
+// One launch holds H half sweeps (H even: whole sweeps) of every lattice. Each half sweep is
+// eight lists of tiles, list x for XCD x: every lattice lives on one XCD (all its tiles, every
+// half sweep), so a tile and the tiles it waits for share one L2; a lattice too large for one
+// XCD's share is cut into eight bands of consecutive tiles, band b on XCD b, and its tiles at a
+// band edge use the device-scope protocol below. Workgroup g runs half sweep h = g / 8P and
+// entry (g % 8P) / 8 of the list of the XCD it runs on (HW_REG_XCC_ID): the dispatcher places
+// workgroup g on XCD (g + r) % 8, so every entry of every list runs exactly once.
+//
+// Ordering. A tile of lattice half sweep k (k = 2 sweep + colour, counted since the batch was
+// made) waits until the tiles of its lattice that hold its neighbour rows have completed k
+// half sweeps; a tile publishes k + 1 when it is done. The same tiles read what this one
+// overwrites, so the wait covers both directions. Waits are for workgroups with lower ids,
+// which each XCD dispatches first: the oldest unfinished workgroup can always proceed. Waits
+// are bounded (BF_SPIN_LIMIT polls): a workgroup that gives up sets a failure bit and exits
+// without touching the lattice, later waiters see the bit and exit at once, the kernel ends
+// normally, and batch_fused_end checks every counter.
+//
+// Memory. Inside one XCD the L2 is the coherence point: stores are acknowledged (s_waitcnt)
+// before the counter is published, every lattice load misses in the CU's L1 (cache policy sc1;
+// a CU may hold a line it read in an earlier half sweep of the launch), and a tile that only
+// tiles of its own XCD wait for publishes with a plain store, which stays in that L2 (a
+// device-scope store costs the poller a trip past the L2: measured 1.4-1.8 us per wait). A
+// band-edge tile of a split lattice adds the device-scope fences: release (L2 write-back)
+// and a device-scope counter store before publishing, acquire (L2 invalidate) after its wait.
+//
+// Moments (MOM). Each tile stores its +1 spins in its slot (parity, colour, tile) before
+// publishing. The last entries of every list are fold entries, one per lattice (one per band
+// and lattice when split): in a colour-1 half sweep it waits for its tiles, sums their slots of
+// both colours and folds M into the sums with atomics (folds of consecutive sweeps may
+// overlap); a split lattice's eight bands add (sum << 8) + 1 to a per-lattice word first and
+// the eighth arrival folds. Then it publishes sweep + 1 in the fold counter of the parity. The
+// colour-0 tiles of sweep s + 2, the next to write those slots, wait for the fold of sweep s.
+constexpr u32 BF_HDR = 64;  // u32 words before the per-tile counters
+constexpr u32 BF_SPIN_LIMIT = 1u << 20;
+constexpr u32 BF_FAIL_WAIT = 1u, BF_FAIL_COUNTERS = 2u;
+constexpr u64 BF_EDGE = 1ull << 63;   // list-entry flag: band-edge tile of a split lattice
+constexpr u64 BF_FOLD = 1ull << 62;   // list-entry flag: fold entry (low word: band)
+constexpr u64 BF_SPLIT = 1ull << 61;  // fold entry of a split lattice
+constexpr u32 BF_LAT_MASK = (1u << 29) - 1u;
+constexpr u32 BF_MAX_SWEEPS = 512;    // sweeps per launch (longer calls take several launches)
+
+struct BatchFusedArgs {
+    const BatchDesc* descs;
+    const u64* seq;   // [8][P] entries (lattice << 32 | tile, flags), BATCH_NO_TILE padding
+    u32* sync;        // [0] failure bits; [BF_HDR + tile] half sweeps completed by the tile
+    u32* fold;        // [2 lattice + parity]: 1 + the last sweep of that parity folded
+    u32* slots;       // [2 parity][2 colour][tiles]: +1 spins written by each tile
+    u64* pacc;        // [2 lattice + parity]: split lattices, (band sums << 8) + arrivals
+    BatchAcc* acc;
+    const unsigned long long* sweeps_done;
+    u32 P, per_half;  // per_half = 8 P workgroups per half sweep
+    u32 total_tiles;
+    FastDiv div_half;
+    u32 one_l2;       // 1: the device runs every workgroup on one XCD
+};
+
+// Wave 0: waits until the `count` tiles first, first + 1, ... (cyclic over the lattice's n
+// tiles, counters at cnt) have completed `target` half sweeps and, when fold_p is set, until
+// *fold_p reaches fold_target. Lanes 0-62 poll one tile each (63 at a time); lane 63 polls the
+// fold counter. False if the wait gave up or another workgroup's did.
+__device__ bool bf_wait(u32* sync, const u32* cnt, u32 n, u32 first, u32 count, u32 target,
+                        const u32* fold_p, u32 fold_target)
+{
+    const u32 lane = threadIdx.x & 63u;
+    u32 spin = 0;
+    for (u32 c0 = 0; c0 < count || (c0 == 0u && fold_p); c0 += 63u) {  // wave-uniform
+        const u32 i = c0 + lane;
+        const bool dep = lane < 63u && i < count;
+        const bool fl = lane == 63u && c0 == 0u && fold_p != nullptr;
+        u32 j = first + (dep ? i : 0u);
+        if (j >= n) j -= n;
+        const u32* p = dep ? cnt + j : fold_p;
+        const u32 tgt = dep ? target : fold_target;
+        bool ok = !(dep || fl);
+        while (true) {
+            if (!ok) ok = (int)(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - tgt) >= 0;
+            if (__ballot(!ok) == 0) break;
+            if ((++spin & 255u) == 0u) {
+                const u32 bad = __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((bad & BF_FAIL_WAIT) || spin >= BF_SPIN_LIMIT) {
+                    if (lane == 0u)
+                        __hip_atomic_fetch_or(sync, BF_FAIL_WAIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return false;
+                }
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    return true;
+}
+
+// Moments of one sweep folded into A with device-scope atomics (128-bit sums by carry).
+__device__ void bf_fold(BatchAcc* A, const BatchDesc& d, u64 plus)
+{
+    const long long m = 2 * (long long)plus - d.n_spins;
+    const u64 m2 = (u64)(m * m);
+    const u64 lo = m2 * m2, hi = __umul64hi(m2, m2);
+    auto add = [](unsigned long long* p, u64 v) {
+        return (u64)__hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    add((unsigned long long*)&A->sum_m, (u64)m);
+    add(&A->sum_abs_m, (u64)(m < 0 ? -m : m));
+    const u64 o2 = add(&A->sum_m2_lo, m2);
+    if (o2 + m2 < o2) add(&A->sum_m2_hi, 1u);
+    const u64 o4 = add(&A->sum_m4_lo, lo);
+    const u64 c4 = hi + (o4 + lo < o4 ? 1u : 0u);
+    if (c4) add(&A->sum_m4_hi, c4);
+    add(&A->samples, 1u);
+}
+
+// Fold entry of lattice `lat`, band `band` (colour-1 half sweeps; wave 0 only).
+__device__ void bf_fold_entry(const BatchFusedArgs& f, const BatchDesc& d, u32 lat, u32 band,
+                              bool split, u32 sw, u32 k)
+{
+    const u32 n = d.n_tiles, par = sw & 1u;
+    const u32 t0 = split ? (u32)((u64)n * band / 8u) : 0u;
+    const u32 t1 = split ? (u32)((u64)n * (band + 1u) / 8u) : n;
+    if (!bf_wait(f.sync, f.sync + BF_HDR + d.tile_begin, n, t0, t1 - t0, k + 1u, nullptr, 0u)) return;
+    const u32* s0 = f.slots + (size_t)(2u * par) * f.total_tiles + d.tile_begin;
+    const u32* s1 = s0 + f.total_tiles;
+    u64 plus = 0;
+    for (u32 t = t0 + (threadIdx.x & 63u); t < t1; t += 64u)
+        plus += (u64)__hip_atomic_load(s0 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
+                __hip_atomic_load(s1 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) plus += __shfl_xor(plus, off, 64);
+    if (threadIdx.x != 0) return;
+    u32* fold = f.fold + 2u * lat + par;
+    if (split) {
+        unsigned long long* pa = (unsigned long long*)f.pacc + 2u * lat + par;
+        const u64 old = (u64)__hip_atomic_fetch_add(pa, (unsigned long long)((plus << 8) + 1u),
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((old & 0xFFu) != 7u) return;  // not the eighth band
+        plus += old >> 8;
+        __hip_atomic_store(pa, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    bf_fold(f.acc + lat, d, plus);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // folded and cleared before publishing
+    // a whole lattice's colour-0 tiles all poll from this XCD: a plain store stays in its L2
+    if (split) __hip_atomic_store(fold, sw + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else __hip_atomic_store(fold, sw + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// K: entries per workgroup. The lists hold groups of K entries: up to K consecutive tiles of
+// one lattice (padded with BATCH_NO_TILE), or one fold entry and padding. A workgroup waits
+// once for the union of its tiles' windows, then runs them in order, publishing each.
+// SET: the tile kinds of this kernel. A batch runs two kernels side by side on two streams
+// (lattices are independent, and each kernel waits only for its own workgroups): the msc
+// lattices with queue-compaction tiles (TILES_QUEUE, 74 VGPRs: 6 workgroups per CU) and the
+// rest (TILES_LIGHT, at most 64 VGPRs: 8 per CU, which the latency-bound int8 tiles need).
+template <bool MOM, u32 K, int SET>
+__global__ __launch_bounds__(256) void batch_fused(BatchFusedArgs f)
+{
+    __shared__ MscQueueLds<2> lds;
+    __shared__ u32 thr_lds[8];
+    __shared__ u32 part[1];  // +1 spins of the current tile
+    __shared__ u32 go;
+    const u32 g = blockIdx.x;
+    const u32 h = fast_div(g, f.div_half);
+    const u32 q = g - h * f.per_half;
+    const u32 x = f.one_l2 ? (q & 7u) : (xcc_id() & 7u);
+    const u64* ents = f.seq + (size_t)x * f.P + (size_t)(q >> 3) * K;
+    if (MOM && threadIdx.x == 0) part[0] = 0;  // ordered before the first add by a barrier below
+    const u64 e0 = ents[0];
+    if (e0 == BATCH_NO_TILE) return;  // padding of a shorter list (whole workgroup)
+    const u32 lat = (u32)(e0 >> 32) & BF_LAT_MASK;
+    const BatchDesc& d = f.descs[lat];
+    const u32 sw = (u32)*f.sweeps_done + (h >> 1), color = h & 1u, par = sw & 1u;
+    if (sw >= d.limit) return;  // past the lattice's sweep limit (nobody waits for it)
+    const u32 k = 2u * sw + color;  // this lattice's half sweeps before this one
+    if (e0 & BF_FOLD) {
+        if (MOM && color == 1u && threadIdx.x < 64u)
+            bf_fold_entry(f, d, lat, (u32)e0, (e0 & BF_SPLIT) != 0, sw, k);
+        return;
+    }
+    u32 kv = 1;  // tiles in this group
+    bool edge_any = (e0 & BF_EDGE) != 0;
+#pragma unroll
+    for (u32 i = 1; i < K; ++i) {
+        const u64 e = ents[i];
+        if (e != BATCH_NO_TILE) {
+            kv = i + 1u;
+            edge_any = edge_any || (e & BF_EDGE) != 0;
+        }
+    }
+    const u32 local0 = (u32)e0 - d.tile_begin;
+    u32* cnt = f.sync + BF_HDR + d.tile_begin;
+    // a launch's first half sweep follows the previous launch (stream order); the fold of
+    // sweep sw - 2 is waited for when that sweep ran in this launch
+    const bool fold_wait = MOM && color == 0u && h >= 4u;
+    if (h > 0u) {
+        if (threadIdx.x < 64u) {
+            const u32 n = d.n_tiles;
+            u32 count = n, first = 0;
+            if (!d.dep_all && kv + 2u * d.dep_span < n) {
+                count = kv + 2u * d.dep_span;
+                first = local0 + n - d.dep_span;
+                if (first >= n) first -= n;
+            }
+            const bool ok = bf_wait(f.sync, cnt, n, first, count, k,
+                                    fold_wait ? f.fold + 2u * lat + par : nullptr, sw - 1u);
+            if (edge_any) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (threadIdx.x == 0) go = ok ? 1u : 0u;
+        }
+        __syncthreads();
+        if (!go) return;  // the whole workgroup: nothing written, counters not advanced
+    }
+    if (MOM && h == 0u) __syncthreads();  // part[0] = 0 before any add (h > 0: the wait's barrier)
+    for (u32 i = 0; i < (K == 1 ? 1u : kv); ++i) {  // block-uniform
+        const u64 e = K == 1 ? e0 : ents[i];
+        const u32 tile = (u32)e, local = tile - d.tile_begin;
+        if (i) __syncthreads();  // the previous tile's threads may still read thr_lds
+        const u32 plus = batch_tile<true, SET>(d, lds, thr_lds, local, d.t0 + sw, color);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are acknowledged
+        if (MOM && plus) atomicAdd(&part[0], plus);  // LDS; part[0] is zero here
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (MOM) {
+                f.slots[(size_t)(2u * par + color) * f.total_tiles + tile] = part[0];
+                part[0] = 0;  // for the next tile (its adds follow the loop's barrier)
+            }
+            if (e & BF_EDGE) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // L2 write-back
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // ... completed before publishing
+                __hip_atomic_store(cnt + local, k + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the slot is written before publishing
+                __hip_atomic_store(cnt + local, k + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
+}
+
+// After a dependency-ordered launch of n sweeps (one workgroup): every tile of a lattice that
+// took part must have published its last half sweep, else a tile went missing (failure bit 2);
+// then the batch's sweep counter advances by n.
+__global__ __launch_bounds__(256) void batch_fused_end(const BatchDesc* descs, const u32* tile_lat,
+                                                       u32 total_tiles, u32* sync,
+                                                       unsigned long long* sweeps_done, u32 n)
+{
+    const u32 sw0 = (u32)sweeps_done[0];
+    int bad = 0;
+    for (u32 t = threadIdx.x; t < total_tiles; t += 256u) {
+        const BatchDesc& d = descs[tile_lat[t]];
+        if (sw0 >= d.limit) continue;
+        const u32 end = (d.limit - sw0 < n) ? d.limit : sw0 + n;
+        if (sync[BF_HDR + t] != 2u * end) bad = 1;
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) {
+        if (bad) sync[0] |= BF_FAIL_COUNTERS;
+        sweeps_done[0] = sw0 + n;
+        sweeps_done[1] = sw0 + n - 1u;
+    }
+}
+
 // ---------------------------------------------------------------------- host side ----
 // IsingBatchLattice, the caller-side description of one lattice, is the C ABI struct of
 // ising_hip.h.
@@ -364,7 +572,20 @@ struct BatchHandle {
     unsigned long long* d_done = nullptr;
     int grid = 0, max_blocks = 0, n_cu = 0, per_cu = 0;
     u32 msc_mode = 0, msc_cap = 4;
-    int persistent = 0;  // 1: one cooperative launch per call; 0: two launches per sweep
+    int persistent = 0;  // 1: dependency-ordered launches; 0: two launches per sweep
+    // dependency-ordered form (batches made by ising_batch_create): the eight lists, P entries
+    // each, and the sync words (header, per-tile counters, fold counters, moment words)
+    // lists of the two kernels in one buffer: the light kernel's 8 x f_P entries, then the
+    // queue kernel's 8 x f_Pq (f_P, f_Pq = 0: no lattices of that kind)
+    u64* d_fseq = nullptr;
+    u32 f_P = 0, f_Pq = 0;
+    hipStream_t f_side = nullptr;  // the queue kernel's stream (default priority)
+    hipEvent_t f_fork = nullptr, f_join = nullptr;
+    u32* d_fsync = nullptr;
+    u32* d_fold = nullptr;
+    u32* d_slots = nullptr;
+    u64* d_pacc = nullptr;
+    size_t fsync_words = 0;  // u64 words of the sync allocation
     int moments = 1;     // launch form: 1 adds the accumulation launch after every sweep
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -402,7 +623,41 @@ static void batch_free(BatchHandle* h)
     (void)hipFree(h->d_plus);
     (void)hipFree(h->d_acc);
     (void)hipFree(h->d_done);
+    (void)hipFree(h->d_fseq);
+    (void)hipFree(h->d_fsync);
+    if (h->f_fork) (void)hipEventDestroy(h->f_fork);
+    if (h->f_join) (void)hipEventDestroy(h->f_join);
+    if (h->f_side) (void)hipStreamDestroy(h->f_side);
     delete h;
+}
+
+// Dependency window of a lattice's tiles (host): tile i holds units [i per_tile, (i + 1)
+// per_tile) of `units` (int8 chunks or msc words), per_row units per row, `rows` rows; a unit
+// reads the other colour at rows up to s away (1 in 2D, one plane = ly rows in 3D, cyclic).
+// Tile i depends on, and is read by, the tiles holding rows r_first - s .. r_last + s of its
+// rows r_first .. r_last: at most `span` tiles away on either side (cyclic). all = 1 when some
+// tile's window covers every tile.
+static void batch_dep_window(u64 units, u64 per_row, u64 per_tile, u64 rows, u64 s, u32 n,
+                             u32& span, u32& all)
+{
+    span = 0;
+    all = 0;
+    for (u32 i = 0; i < n; ++i) {
+        const u64 c0 = (u64)i * per_tile;
+        const u64 c1 = (c0 + per_tile < units ? c0 + per_tile : units) - 1u;
+        const u64 ra = c0 / per_row, rb = c1 / per_row;
+        if (rb - ra + 2u * s + 1u >= rows) {
+            all = 1;
+            return;
+        }
+        const u64 lo = (ra + rows - s) % rows, hi = (rb + s) % rows;
+        const u32 t_lo = (u32)(lo * per_row / per_tile);
+        const u32 t_hi = (u32)(((hi + 1u) * per_row - 1u) / per_tile);
+        const u32 back = (i + n - t_lo) % n, fwd = (t_hi + n - i) % n;
+        span = back > span ? back : span;
+        span = fwd > span ? fwd : span;
+    }
+    if (2ull * span + 1u >= n) all = 1;
 }
 
 // Host-side descriptors and tile map of n lattices (no device calls). On success `descs` holds
@@ -462,6 +717,8 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
             d.g.div_ly = make_fastdiv(d.g.ly);
             for (int k = 0; k <= 2 * L.ndim; ++k) d.tb.v[k] = L.table[k];
             d.n_tiles = (d.g.n_chunks + 255u) / 256u;
+            batch_dep_window(d.g.n_chunks, d.g.chunks_row, 256u, (u64)rows,
+                             L.ndim == 2 ? 1u : (u64)L.ly, d.n_tiles, d.dep_span, d.dep_all);
         } else if (L.kind >= BATCH_MSC && L.kind <= BATCH_MSC_QUEUE) {
             if (L.ndim != 2 || L.lz != 1 || L.lx % 128) return ising_fail(ISING_EINVAL, "batch: msc needs 2D, Lx % 128 == 0");
             if ((long long)L.ly * (L.lx / 128) + (1ll << 20) > 0xFFFFFFFFll)
@@ -475,6 +732,7 @@ static int batch_build(const IsingBatchLattice* lats, int n, u32 msc_mode,
             d.layout = (u32)msc_layout(d.nw, d.n_words);
             const u32 per = d.msc_queue ? 512u : 256u;
             d.n_tiles = (d.n_words + per - 1u) / per;
+            batch_dep_window(d.n_words, d.nw, per, d.rows, 1u, d.n_tiles, d.dep_span, d.dep_all);
         } else {
             return ising_fail(ISING_EINVAL, "batch: kind must be 0 (int8), 1 (msc), 2 (msc lockstep) or 3 (msc queue)");
         }
@@ -533,6 +791,103 @@ static void batch_schedule(const std::vector<BatchDesc>& descs, u32 k, std::vect
             seq[((i / k) * X + x) * k + i % k] = lists[x][i];
 }
 
+// Lists of the dependency-ordered form: seq[x * P + j] is entry j of XCD x's list for every
+// half sweep. Lattices are placed whole on one XCD, largest first onto the least loaded one
+// (cost = tiles x an estimate of a tile's time); a lattice whose cost exceeds half an XCD's
+// share, whose window is not the whole lattice and whose bands keep interior tiles is cut into
+// eight bands of consecutive tiles, band b on XCD b, and its tiles whose window leaves their
+// band carry BF_EDGE. Within a list, lattices with longer tiles come first. P = the longest
+// list; shorter ones are padded with BATCH_NO_TILE.
+static u32 batch_fused_cost(const BatchDesc& d)
+{
+    if (d.kind == BATCH_INT8) return d.vec / 4u;  // 1024 .. 4096 sites
+    return d.msc_queue ? 12u : 8u;                // 512 / 256 words, about 3x / 2x an int8 tile
+}
+
+static bool batch_queue_lattice(const BatchDesc& d)
+{
+    return d.kind != BATCH_INT8 && d.msc_queue;
+}
+
+// The lists of one of the two kernels: queue = the msc lattices with queue-compaction tiles,
+// else every other lattice. P = 0 when the kernel has no lattices.
+static void batch_fused_schedule(const std::vector<BatchDesc>& descs, bool queue, u32 K,
+                                 std::vector<u64>& seq, u32& P)
+{
+    constexpr u32 X = 8;
+    u64 total = 0;
+    std::vector<u32> order;
+    for (u32 i = 0; i < (u32)descs.size(); ++i) {
+        if (batch_queue_lattice(descs[i]) != queue) continue;
+        order.push_back(i);
+        total += (u64)descs[i].n_tiles * batch_fused_cost(descs[i]);
+    }
+    seq.clear();
+    P = 0;
+    if (order.empty()) return;
+    std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+        return (u64)descs[a].n_tiles * batch_fused_cost(descs[a]) >
+               (u64)descs[b].n_tiles * batch_fused_cost(descs[b]);
+    });
+    u64 load[X] = {};
+    std::vector<int> home(descs.size(), -1);  // XCD of a whole lattice, -1 = split into bands
+    for (u32 li : order) {
+        const BatchDesc& d = descs[li];
+        const u64 cost = (u64)d.n_tiles * batch_fused_cost(d);
+        const bool split = cost * 2u * X > total && !d.dep_all && d.n_tiles / X >= 2u * d.dep_span + 2u;
+        if (split) {
+            for (u32 x = 0; x < X; ++x) load[x] += cost / X;
+            continue;
+        }
+        u32 best = 0;
+        for (u32 x = 1; x < X; ++x) best = load[x] < load[best] ? x : best;
+        home[li] = (int)best;
+        load[best] += cost;
+    }
+    std::vector<u32> by_tile(order);
+    std::stable_sort(by_tile.begin(), by_tile.end(), [&](u32 a, u32 b) {
+        return batch_fused_cost(descs[a]) > batch_fused_cost(descs[b]);
+    });
+    std::vector<u64> lists[X];
+    auto pad = [&](std::vector<u64>& l) {
+        while (l.size() % K) l.push_back(BATCH_NO_TILE);
+    };
+    for (u32 li : by_tile) {
+        const BatchDesc& d = descs[li];
+        if (home[li] >= 0) {
+            for (u32 t = 0; t < d.n_tiles; ++t) lists[home[li]].push_back(((u64)li << 32) | (d.tile_begin + t));
+            pad(lists[home[li]]);
+            continue;
+        }
+        for (u32 b = 0; b < X; ++b) {
+            const u32 t0 = (u32)((u64)d.n_tiles * b / X), t1 = (u32)((u64)d.n_tiles * (b + 1) / X);
+            for (u32 t = t0; t < t1; ++t) {
+                const bool edge = t < t0 + d.dep_span || t + d.dep_span >= t1;
+                lists[b].push_back(((u64)li << 32) | (d.tile_begin + t) | (edge ? BF_EDGE : 0ull));
+            }
+            pad(lists[b]);
+        }
+    }
+    // fold entries last: one per whole lattice on its XCD, one per band of a split lattice
+    for (u32 li : order) {
+        if (home[li] >= 0) {
+            lists[home[li]].push_back(((u64)li << 32) | BF_FOLD);
+            pad(lists[home[li]]);
+        } else {
+            for (u32 b = 0; b < X; ++b) {
+                lists[b].push_back(((u64)li << 32) | BF_FOLD | BF_SPLIT | b);
+                pad(lists[b]);
+            }
+        }
+    }
+    size_t longest = 0;
+    for (u32 x = 0; x < X; ++x) longest = lists[x].size() > longest ? lists[x].size() : longest;
+    P = (u32)longest;
+    seq.assign((size_t)X * P, BATCH_NO_TILE);
+    for (u32 x = 0; x < X; ++x)
+        for (size_t i = 0; i < lists[x].size(); ++i) seq[(size_t)x * P + i] = lists[x][i];
+}
+
 // Capacity of the launch order for cap_lat lattices and cap_tiles tiles: each lattice can
 // lengthen the longest list by at most one tile beyond its even share, and k tiles per
 // workgroup pad each list by at most 7 more.
@@ -548,7 +903,7 @@ static int batch_alloc(BatchHandle* h, u32 cap_lat, u32 cap_tiles, int blocks_pe
     int dev = 0;
     ISING_HIP(hipGetDevice(&dev));
     ISING_HIP(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    ISING_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&h->max_blocks, (const void*)batch_persistent, 256, 0));
+    ISING_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&h->max_blocks, (const void*)batch_half<1>, 256, 0));
     if (h->max_blocks < 1) return ising_fail(ISING_EHIP, "batch: kernel does not fit on a CU");
     h->per_cu = (blocks_per_cu <= 0 || blocks_per_cu > h->max_blocks) ? h->max_blocks : blocks_per_cu;
     h->cap_lat = cap_lat;
@@ -584,6 +939,125 @@ static int batch_zero_stats(BatchHandle* h, hipStream_t st)
     return ISING_OK;
 }
 
+// Lists and sync words of the dependency-ordered form for the handle's lattices (synchronous;
+// batches made by ising_batch_create).
+// Builds and uploads the two kernels' lists for k tiles per workgroup (synchronous).
+static int batch_fused_lists(BatchHandle* h, u32 k)
+{
+    std::vector<u64> light, queue;
+    batch_fused_schedule(h->host_descs, false, k, light, h->f_P);
+    batch_fused_schedule(h->host_descs, true, 1u, queue, h->f_Pq);
+    light.insert(light.end(), queue.begin(), queue.end());
+    if (h->d_fseq) ISING_HIP(hipFree(h->d_fseq));
+    h->d_fseq = nullptr;
+    ISING_HIP(hipMalloc((void**)&h->d_fseq, sizeof(u64) * light.size()));
+    ISING_HIP(hipMemcpy(h->d_fseq, light.data(), sizeof(u64) * light.size(), hipMemcpyHostToDevice));
+    return ISING_OK;
+}
+
+static int batch_fused_alloc(BatchHandle* h, const std::vector<BatchDesc>& descs, u32 tiles)
+{
+    const size_t n = descs.size();
+    // header + counters, fold counters, slots (u32), split-lattice words (u64), in u64 units
+    const size_t cnt_words = (BF_HDR + 5 * (size_t)tiles + 2 * n + 1) / 2;
+    h->fsync_words = cnt_words + 2 * n;
+    ISING_HIP(hipMalloc((void**)&h->d_fsync, 8 * h->fsync_words));
+    h->d_fold = h->d_fsync + BF_HDR + tiles;
+    h->d_slots = h->d_fold + 2 * n;
+    h->d_pacc = (u64*)h->d_fsync + cnt_words;
+    ISING_HIP(hipStreamCreateWithFlags(&h->f_side, hipStreamNonBlocking));
+    ISING_HIP(hipEventCreateWithFlags(&h->f_fork, hipEventDisableTiming));
+    ISING_HIP(hipEventCreateWithFlags(&h->f_join, hipEventDisableTiming));
+    return batch_fused_lists(h, h->per_wg);
+}
+
+static void batch_zero_fused(BatchHandle* h, hipStream_t st)
+{
+    if (!h->d_fsync) return;
+    const u32 words = (u32)h->fsync_words;
+    const u32 blocks = words / 256u + 1u < 1024u ? words / 256u + 1u : 1024u;
+    hipLaunchKernelGGL(batch_zero_u64, dim3(blocks), dim3(256), 0, st, (unsigned long long*)h->d_fsync, words);
+}
+
+static void batch_launch_fused_kernel(u32 K, bool mom, bool queue, dim3 grid, hipStream_t st,
+                                      const BatchFusedArgs& f)
+{
+#define BF_LAUNCH(KK, SET)                                                                       \
+    if (mom) hipLaunchKernelGGL((batch_fused<true, KK, SET>), grid, dim3(256), 0, st, f);       \
+    else hipLaunchKernelGGL((batch_fused<false, KK, SET>), grid, dim3(256), 0, st, f)
+    if (queue) {
+        BF_LAUNCH(1, TILES_QUEUE);
+        return;
+    }
+    switch (K) {
+    case 2: BF_LAUNCH(2, TILES_LIGHT); break;
+    case 4: BF_LAUNCH(4, TILES_LIGHT); break;
+    case 8: BF_LAUNCH(8, TILES_LIGHT); break;
+    default: BF_LAUNCH(1, TILES_LIGHT); break;
+    }
+#undef BF_LAUNCH
+}
+
+// A launch's half sweeps hold fewer than 2^32 work-items.
+static u32 batch_fused_per_launch(u32 per_half)
+{
+    u32 per_launch = BF_MAX_SWEEPS;
+    while (per_launch > 1u && 2ull * per_launch * per_half * 256u >= (1ull << 32)) per_launch >>= 1;
+    return per_launch;
+}
+
+// Each chunk of sweeps: the light kernel on `st` and the queue kernel on the side stream, forked
+// and joined with events (captured as two parallel graph nodes), then the counter check.
+static int batch_launch_fused(BatchHandle* h, u32 n_sweeps, hipStream_t st)
+{
+    int place = -1;
+    if (int rc = fused_placement(&place)) return rc;
+    if (place < 0) return ising_fail(ISING_EINVAL, "batch: the dependency-ordered form needs round-robin or single-XCD placement");
+    BatchFusedArgs f;
+    f.descs = h->d_descs;
+    f.sync = h->d_fsync;
+    f.pacc = h->d_pacc;
+    f.fold = h->d_fold;
+    f.slots = h->d_slots;
+    f.total_tiles = h->total_tiles;
+    f.acc = h->d_acc;
+    f.sweeps_done = h->d_done;
+    f.one_l2 = place == 0 ? 1u : 0u;
+    BatchFusedArgs fl = f, fq = f;
+    fl.seq = h->d_fseq;
+    fl.P = h->f_P;
+    fl.per_half = 8u * (h->f_P / h->per_wg);
+    fq.seq = h->d_fseq + 8ull * h->f_P;
+    fq.P = h->f_Pq;
+    fq.per_half = 8u * h->f_Pq;
+    fl.div_half = make_fastdiv(fl.per_half ? fl.per_half : 1u);
+    fq.div_half = make_fastdiv(fq.per_half ? fq.per_half : 1u);
+    const u32 per_launch = batch_fused_per_launch(fl.per_half > fq.per_half ? fl.per_half : fq.per_half);
+    if (2ull * per_launch * (fl.per_half > fq.per_half ? fl.per_half : fq.per_half) * 256u >= (1ull << 32))
+        return ising_fail(ISING_ERANGE, "batch: too many tiles per half sweep for one launch");
+    const bool moms = h->moments != 0;
+    for (u32 done = 0; done < n_sweeps;) {
+        const u32 k = n_sweeps - done < per_launch ? n_sweeps - done : per_launch;
+        const bool both = fl.P && fq.P;
+        if (both) {
+            ISING_HIP(hipEventRecord(h->f_fork, st));
+            ISING_HIP(hipStreamWaitEvent(h->f_side, h->f_fork, 0));
+        }
+        if (fq.P)
+            batch_launch_fused_kernel(1u, moms, true, dim3(2u * k * fq.per_half), both ? h->f_side : st, fq);
+        if (fl.P) batch_launch_fused_kernel(h->per_wg, moms, false, dim3(2u * k * fl.per_half), st, fl);
+        if (both) {
+            ISING_HIP(hipEventRecord(h->f_join, h->f_side));
+            ISING_HIP(hipStreamWaitEvent(st, h->f_join, 0));
+        }
+        hipLaunchKernelGGL(batch_fused_end, dim3(1), dim3(256), 0, st, (const BatchDesc*)h->d_descs,
+                           (const u32*)h->d_tile_lat, h->total_tiles, h->d_fsync, h->d_done, k);
+        done += k;
+    }
+    ISING_LAUNCHED("batch_fused");
+    return ISING_OK;
+}
+
 static void batch_launch_half(u32 per_wg, dim3 grid, hipStream_t st, const BatchKernelArgs& ka,
                               u32 color, u32 acc_prev)
 {
@@ -613,12 +1087,7 @@ static int batch_launch(BatchHandle* h, u32 n_sweeps, hipStream_t st)
     ka.n_sweeps = n_sweeps;
     ka.msc_mode = h->msc_mode;
     ka.msc_cap = h->msc_cap;
-    if (h->persistent) {
-        void* args[] = {&ka};
-        ISING_HIP(hipLaunchCooperativeKernel((const void*)batch_persistent, dim3(h->grid),
-                                             dim3(256), args, 0, st));
-        return ISING_OK;
-    }
+    if (h->persistent) return batch_launch_fused(h, n_sweeps, st);
     for (u32 s = 0; s < n_sweeps; ++s) {
         const u32 acc_prev = (h->moments && s > 0) ? 1u : 0u;
         batch_launch_half(h->per_wg, dim3(h->n_seq + (acc_prev ? h->n_lat : 0u)), st, ka, 0u,
@@ -643,7 +1112,7 @@ extern "C" {
 int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, int msc_mode,
                        int msc_cap, void** out)
 {
-    if (!lats || n <= 0 || !out) return ising_fail(ISING_EINVAL, "batch_create: bad argument");
+    if (!lats || n <= 0 || n > (int)BF_LAT_MASK || !out) return ising_fail(ISING_EINVAL, "batch_create: bad argument");
     if (int rc = batch_check_modes(msc_mode, msc_cap)) return rc;
     *out = nullptr;
     std::vector<BatchDesc> descs;
@@ -664,6 +1133,12 @@ int ising_batch_create(const IsingBatchLattice* lats, int n, int blocks_per_cu, 
     }
     batch_set_sizes(h, (u32)n, tiles, (u32)seq.size());
     h->host_descs = descs;
+    rc = batch_fused_alloc(h, descs, tiles);
+    if (rc) {
+        batch_free(h);
+        return rc;
+    }
+    batch_zero_fused(h, 0);
     hipError_t e = hipMemcpy(h->d_descs, descs.data(), sizeof(BatchDesc) * (size_t)n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_tile_lat, tile_lat.data(), sizeof(u32) * tile_lat.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_tile_seq, seq.data(), sizeof(u64) * seq.size(), hipMemcpyHostToDevice);
@@ -766,7 +1241,7 @@ int ising_batch_assign(void* handle, const IsingBatchLattice* lats, int n, void*
     return batch_zero_stats(h, st);
 }
 
-// n_sweeps sweeps of every lattice, one cooperative launch (stream-ordered, asynchronous).
+// n_sweeps sweeps of every lattice in the batch's form (stream-ordered, asynchronous).
 int ising_batch_sweeps(void* handle, long long n_sweeps, void* stream)
 {
     BatchHandle* h = (BatchHandle*)handle;
@@ -775,9 +1250,8 @@ int ising_batch_sweeps(void* handle, long long n_sweeps, void* stream)
     return batch_launch(h, (u32)n_sweeps, as_stream(stream));
 }
 
-// Capture one launch of `sweeps_per_replay` sweeps into a hipGraph (stream capture); replays
-// continue the device-side sweep counter. Returns ISING_EHIP if capture is not supported for
-// the cooperative launch on this runtime (callers then use ising_batch_sweeps).
+// Capture the launches of `sweeps_per_replay` sweeps into a hipGraph (stream capture); replays
+// continue the device-side sweep counter.
 int ising_batch_graph(void* handle, long long sweeps_per_replay, void* stream)
 {
     BatchHandle* h = (BatchHandle*)handle;
@@ -810,8 +1284,8 @@ int ising_batch_replay(void* handle, int replays, void* stream)
     return ISING_OK;
 }
 
-// Accumulators: n x 6 int64 (sum M, sum |M|, sum M^2, sum M^4 lo, sum M^4 hi, samples) and the
-// number of sweeps done. Synchronous.
+// Accumulators: n x 7 int64 (sum M, sum |M|, sum M^2 lo, sum M^2 hi, sum M^4 lo, sum M^4 hi,
+// samples) and the number of sweeps done. Synchronous.
 int ising_batch_stats(void* handle, long long* out, long long* sweeps_done, void* stream)
 {
     BatchHandle* h = (BatchHandle*)handle;
@@ -837,22 +1311,48 @@ int ising_batch_reset_stats(void* handle, void* stream)
     return ISING_OK;
 }
 
-// 1 = one persistent cooperative launch per call (grid barrier between half sweeps),
-// 0 = one launch per half sweep plus one accumulation launch per sweep (the default).
-// Both give identical lattices and sums. Destroys a captured graph.
+// 1 = dependency-ordered launches (up to 512 sweeps per launch, tiles ordered by per-tile
+// completion counters; batches made by ising_batch_create, on a device whose dispatches place
+// workgroups round-robin over its XCDs or all on one), 0 = one launch per half sweep plus one
+// accumulation launch per sweep (the default). Both give identical lattices and sums. Destroys
+// a captured graph.
 int ising_batch_set_persistent(void* handle, int persistent)
 {
     BatchHandle* h = (BatchHandle*)handle;
     if (!h || (persistent != 0 && persistent != 1))
         return ising_fail(ISING_EINVAL, "batch_set_persistent: bad argument");
+    if (persistent) {
+        if (!h->d_fsync)
+            return ising_fail(ISING_EINVAL, "batch_set_persistent: the dependency-ordered form needs a batch made by ising_batch_create");
+        int place = -1;  // probed here, outside any stream capture
+        if (int rc = fused_placement(&place)) return rc;
+        if (place < 0)
+            return ising_fail(ISING_EINVAL, "batch_set_persistent: this device places workgroups neither round-robin nor on one XCD");
+    }
     batch_drop_graph(h);
     h->persistent = persistent;
     return ISING_OK;
 }
 
+// Failure bits of the dependency-ordered launches since the batch was made (synchronizes the
+// stream): 1 = a tile's wait gave up, 2 = a tile did not publish its last half sweep. The
+// lattices and sums are undefined after a failure.
+int ising_batch_status(void* handle, int* failed, void* stream)
+{
+    BatchHandle* h = (BatchHandle*)handle;
+    if (!h || !failed) return ising_fail(ISING_EINVAL, "batch_status: bad argument");
+    *failed = 0;
+    if (!h->d_fsync) return ISING_OK;
+    u32 bits = 0;
+    ISING_HIP(hipMemcpyAsync(&bits, h->d_fsync, sizeof(bits), hipMemcpyDeviceToHost, as_stream(stream)));
+    ISING_HIP(hipStreamSynchronize(as_stream(stream)));
+    *failed = (int)bits;
+    return ISING_OK;
+}
+
 // 1 (default) = accumulate the moments of M after every sweep; 0 = skip that work (one launch
-// per sweep fewer in the launch form; the persistent form always accumulates). Lattices are
-// the same either way. Destroys a captured graph.
+// per sweep fewer in the launch form, no per-tile atomics in the dependency-ordered form).
+// Lattices are the same either way. Destroys a captured graph.
 int ising_batch_set_moments(void* handle, int on)
 {
     BatchHandle* h = (BatchHandle*)handle;
@@ -862,9 +1362,10 @@ int ising_batch_set_moments(void* handle, int on)
     return ISING_OK;
 }
 
-// Launch form: each tile workgroup runs k (1, 2, 4 or 8) consecutive tiles of its XCD's list in
-// sequence (fewer, longer-lived workgroups; one tile each by default). Rebuilds and uploads the
-// launch order after a device synchronisation (no launch may be in flight on any stream);
+// Each tile workgroup runs k (1, 2, 4 or 8) consecutive tiles of its XCD's list in sequence
+// (fewer, longer-lived workgroups; one tile each by default); in the dependency-ordered form
+// the k tiles are consecutive tiles of one lattice and share one wait. Rebuilds and uploads the
+// launch orders after a device synchronisation (no launch may be in flight on any stream);
 // results do not depend on k. Destroys a captured graph.
 int ising_batch_set_tiles_per_workgroup(void* handle, int k)
 {
@@ -883,6 +1384,7 @@ int ising_batch_set_tiles_per_workgroup(void* handle, int k)
     }
     h->per_wg = (u32)k;
     if (h->n_lat) batch_set_sizes(h, h->n_lat, h->total_tiles, (u32)(seq.size() / (u32)k));
+    if (h->d_fseq) return batch_fused_lists(h, (u32)k);  // the dependency-ordered form's lists
     return ISING_OK;
 }
 

@@ -48,6 +48,44 @@ __device__ __forceinline__ Vec<V> load_vec(const int8_t* p)
     return r;
 }
 
+// Loads that always miss in the CU's L1 (device-scope cache policy sc1, served by the L2): for
+// kernels in which another workgroup of the same launch may have rewritten the bytes after this
+// CU cached them (the dependency-ordered batch). base is wave-uniform; off < 2^32.
+constexpr int COH_SC1 = 16;  // buffer-instruction cache policy bit of gfx950: sc1
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t coh_rsrc(const void* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000);
+}
+
+template <int V, bool COH>
+__device__ __forceinline__ Vec<V> load_vec_at(const int8_t* base, u64 off)
+{
+    if constexpr (!COH) {
+        return load_vec<V>(base + off);
+    } else {
+        const __amdgpu_buffer_rsrc_t r = coh_rsrc(base);
+        Vec<V> v;
+        if constexpr (V == 16) {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, (u32)off, 0, COH_SC1);
+            v.w[0] = q[0]; v.w[1] = q[1]; v.w[2] = q[2]; v.w[3] = q[3];
+        } else if constexpr (V == 8) {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, (u32)off, 0, COH_SC1);
+            v.w[0] = q[0]; v.w[1] = q[1];
+        } else {
+            v.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, (u32)off, 0, COH_SC1);
+        }
+        return v;
+    }
+}
+
+template <bool COH>
+__device__ __forceinline__ u32 load_byte_at(const int8_t* base, u64 off)
+{
+    if constexpr (!COH) return (u32)(uint8_t)base[off];
+    else return (u32)__builtin_amdgcn_raw_buffer_load_b8(coh_rsrc(base), (u32)off, 0, COH_SC1);
+}
+
 template <int V>
 __device__ __forceinline__ void store_vec(int8_t* p, const Vec<V>& r)
 {
@@ -94,7 +132,8 @@ __device__ __forceinline__ void int8_fill_table(u32* thr_lds, const Int8Table& t
 // ((b * 0x204081) & 0x01010101 moves bit b to byte b). Metropolis adds the forced flips
 // (count >= d) by a bytewise compare and flips with s ^ 0xFE; heat bath writes 0x01 or 0xFF.
 // Returns the number of -1 spins written (for magnetisation sums when COUNT).
-template <int NDIM, int HB, int V, bool COUNT>
+// COH: the other colour and the own sites are read with loads that miss in the L1.
+template <int NDIM, int HB, int V, bool COUNT, bool COH = false>
 __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
                                                  const int8_t* __restrict__ other, const Int8Geom& g,
                                                  const u32* thr_lds, u32 t, u32 color, u32 k0,
@@ -111,22 +150,22 @@ __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
     const u64 rbase = (u64)row * g.lh;
     const u64 n0 = rbase + j0;
 
-    const Vec<V> S = load_vec<V>(own + n0);
-    const Vec<V> C = load_vec<V>(other + n0);
+    const Vec<V> S = load_vec_at<V, COH>(own, n0);
+    const Vec<V> C = load_vec_at<V, COH>(other, n0);
     const u32 yu = (y + 1 == g.ly) ? 0u : y + 1;
     const u32 yd = (y == 0) ? g.ly - 1 : y - 1;
-    const Vec<V> U = load_vec<V>(other + ((u64)z * g.ly + yu) * g.lh + j0);
-    const Vec<V> D = load_vec<V>(other + ((u64)z * g.ly + yd) * g.lh + j0);
+    const Vec<V> U = load_vec_at<V, COH>(other, ((u64)z * g.ly + yu) * g.lh + j0);
+    const Vec<V> D = load_vec_at<V, COH>(other, ((u64)z * g.ly + yd) * g.lh + j0);
     Vec<V> ZU, ZD;
     if constexpr (NDIM == 3) {
         const u32 zu = (z + 1 == g.lz) ? 0u : z + 1;
         const u32 zd = (z == 0) ? g.lz - 1 : z - 1;
-        ZU = load_vec<V>(other + ((u64)zu * g.ly + y) * g.lh + j0);
-        ZD = load_vec<V>(other + ((u64)zd * g.ly + y) * g.lh + j0);
+        ZU = load_vec_at<V, COH>(other, ((u64)zu * g.ly + y) * g.lh + j0);
+        ZD = load_vec_at<V, COH>(other, ((u64)zd * g.ly + y) * g.lh + j0);
     }
     // horizontal neighbour: j + 1 when p == 1, j - 1 when p == 0 (one edge byte from memory)
     const u32 je = p ? ((j0 + V == g.lh) ? 0u : j0 + V) : ((j0 == 0) ? g.lh - 1 : j0 - 1);
-    const u32 e = (u32)(uint8_t)other[rbase + je];
+    const u32 e = load_byte_at<COH>(other, rbase + je);
     Vec<V> H;
     if (p) {
 #pragma unroll

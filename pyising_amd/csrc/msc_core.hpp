@@ -147,6 +147,7 @@ struct MscSegLoad {
     u32 right;  // per lane: 1 = colour sites at odd columns, neighbour j + 1
 };
 
+template <bool COH = false>
 __device__ __forceinline__ MscSegLoad msc_load_seg(const MscArgs& a, u32 w0, u32 lane)
 {
     w0 = uni(w0);
@@ -156,10 +157,10 @@ __device__ __forceinline__ MscSegLoad msc_load_seg(const MscArgs& a, u32 w0, u32
     const u32 yu = (y + 1u == a.rows) ? 0u : y + 1u;
     const u32 yd = (y == 0u) ? a.rows - 1u : y - 1u;
     MscSegLoad L;
-    L.o = (a.own + w0)[lane];
-    L.c = (a.other + w0)[lane];
-    L.up = a.other[(yu << lg) + x];
-    L.dn = a.other[(yd << lg) + x];
+    L.o = msc_ld<COH>(a.own + w0 + lane);
+    L.c = msc_ld<COH>(a.other + w0 + lane);
+    L.up = msc_ld<COH>(a.other + (yu << lg) + x);
+    L.dn = msc_ld<COH>(a.other + (yd << lg) + x);
     L.right = (y + a.color) & 1u;
     return L;
 }
@@ -194,6 +195,7 @@ struct MscLoad {
     u64 o, c, hz, up, dn;
 };
 
+template <bool COH = false>
 __device__ __forceinline__ MscLoad msc_load(const MscArgs& a, u32 w)
 {
     const u32 y = fast_div(w, a.div_nw);
@@ -201,15 +203,15 @@ __device__ __forceinline__ MscLoad msc_load(const MscArgs& a, u32 w)
     const u64 rbase = (u64)y * a.nw;
     const u32 p = (y + a.color) & 1u;
     MscLoad L;
-    L.o = a.own[w];
-    L.c = a.other[w];
+    L.o = msc_ld<COH>(a.own + w);
+    L.c = msc_ld<COH>(a.other + w);
     const u32 xn = p ? ((x + 1 == a.nw) ? 0u : x + 1) : ((x == 0) ? a.nw - 1 : x - 1);
-    const u64 nb = a.other[rbase + xn];
+    const u64 nb = msc_ld<COH>(a.other + rbase + xn);
     L.hz = p ? ((L.c >> 1) | (nb << 63)) : ((L.c << 1) | (nb >> 63));
     const u32 yu = (y + 1 == a.rows) ? 0u : y + 1;
     const u32 yd = (y == 0) ? a.rows - 1 : y - 1;
-    L.up = a.other[(u64)yu * a.nw + x];
-    L.dn = a.other[(u64)yd * a.nw + x];
+    L.up = msc_ld<COH>(a.other + (u64)yu * a.nw + x);
+    L.dn = msc_ld<COH>(a.other + (u64)yd * a.nw + x);
     return L;
 }
 
@@ -226,13 +228,13 @@ __device__ __forceinline__ MscState msc_classify(const MscArgs& a, u32 w)
 // Classification of word w in a kernel whose waves hold 64 consecutive words starting at a
 // multiple of 64 (all lanes of the wave must call it when LAY != MSC_GENERAL; with n_words a
 // multiple of 64 a wave is wholly valid or wholly past the end, so callers branch per wave).
-template <int LAY>
+template <int LAY, bool COH = false>
 __device__ __forceinline__ MscState msc_classify_wave(const MscArgs& a, u32 w)
 {
     const u32 lane = threadIdx.x & 63u;
-    if constexpr (LAY == MSC_ROWW) return msc_classify_row(msc_load_row(a, w - lane, lane));
-    else if constexpr (LAY == MSC_SEGW) return msc_classify_seg(msc_load_seg(a, w - lane, lane), a.nw, lane);
-    else return msc_classify(a, w);
+    if constexpr (LAY == MSC_ROWW) return msc_classify_row(msc_load_row<COH>(a, w - lane, lane));
+    else if constexpr (LAY == MSC_SEGW) return msc_classify_seg(msc_load_seg<COH>(a, w - lane, lane), a.nw, lane);
+    else return msc_classify(msc_load<COH>(a, w));
 }
 
 // Philox call of word w at level pair lp (lp wave-uniform): counter (w, t, colour, 1 + lp).
@@ -467,7 +469,6 @@ struct MscTileLoads {
 template <int R, int LAY, bool COH = false>
 __device__ __forceinline__ void msc_tile_load(const MscArgs& a, u32 base, MscTileLoads<R, LAY>& T)
 {
-    static_assert(!COH || LAY == MSC_ROWW, "coherent loads are implemented for whole-row waves");
     const u32 tid = threadIdx.x;
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
@@ -476,10 +477,10 @@ __device__ __forceinline__ void msc_tile_load(const MscArgs& a, u32 base, MscTil
             const u32 w0 = base + (u32)rr * 256u + (tid & ~63u);
             const u32 wc = w0 < a.n_words ? w0 : a.n_words - 64u;
             if (LAY == MSC_ROWW) T.lr[rr] = msc_load_row<COH>(a, wc, tid & 63u);
-            else T.ls[rr] = msc_load_seg(a, wc, tid & 63u);
+            else T.ls[rr] = msc_load_seg<COH>(a, wc, tid & 63u);
         } else {
             const u32 w = base + (u32)rr * 256u + tid;
-            T.lg[rr] = msc_load(a, w < a.n_words ? w : a.n_words - 1u);
+            T.lg[rr] = msc_load<COH>(a, w < a.n_words ? w : a.n_words - 1u);
         }
     }
 }
@@ -674,13 +675,13 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
     return ones;
 }
 
-template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT>
+template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, bool COH = false>
 __device__ __forceinline__ u32 msc_queue_tile(const MscArgs& a, u64 codes, u32 base,
                                               MscQueueLds<R>& L, u32& calls, u32& slots,
                                               u32& words)
 {
     MscTileLoads<R, LAY> T;
-    msc_tile_load<R, LAY>(a, base, T);
+    msc_tile_load<R, LAY, COH>(a, base, T);
     return msc_queue_tile_loaded<R, CAP, LAY, STATS, COUNT, NT>(a, codes, base, L, T,
                                                              MscNoPrefetch{}, calls, slots, words);
 }

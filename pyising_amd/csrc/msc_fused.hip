@@ -38,6 +38,7 @@
 // placement an XCD would receive more or fewer than its band's tiles: a tile would go missing,
 // which the waits (bounded) or the host's check of the counters after the launch reports.
 #include "msc_core.hpp"
+#include "placement.hpp"
 
 #include <mutex>
 #include <vector>
@@ -64,11 +65,6 @@ struct FusedArgs {
     u32 per_band;   // tiles per XCD band
     u32 one_l2;     // 1: every workgroup runs on one XCD (one L2); 0: round-robin over 8
 };
-
-__device__ __forceinline__ u32 xcc_id()
-{
-    return __builtin_amdgcn_s_getreg((15 << 11) | (0 << 6) | 20) & 15u;  // HW_REG_XCC_ID
-}
 
 // Wave 0 waits until the tiles this one depends on (lanes 0-2: `dep`; `seen` = the value each
 // polling lane loaded first) have completed `target` half sweeps. The tile's own loads are
@@ -175,7 +171,7 @@ static int g_placement[64];
 static std::once_flag g_placement_init;
 static std::mutex g_placement_mu;
 
-static int fused_placement(int* out)
+int fused_placement(int* out)
 {
     std::call_once(g_placement_init, [] {
         for (int& v : g_placement) v = -2;

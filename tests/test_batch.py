@@ -1,5 +1,6 @@
-"""Batched temperature scans: every lattice of a persistent batched launch is bit-identical to
-the same lattice swept alone, and the device-side Binder sums are exact."""
+"""Batched temperature scans: every lattice of a batched launch (one launch per half sweep, or
+dependency-ordered launches of many sweeps) is bit-identical to the same lattice swept alone and
+to the oracle, and the device-side Binder sums are exact."""
 
 import ctypes as C
 
@@ -73,26 +74,42 @@ def test_batch_bitwise_equals_standalone(hip, msc_mode, persistent):
 
 
 @pytest.mark.gpu
-def test_persistent_tiny_grid_orders_the_colours(hip):
-    """The persistent form with one workgroup per CU: every workgroup runs several tiles per
-    half sweep, so colour 1 of a lattice is correct only if the grid barrier really waits for
-    every colour-0 tile (the race a checkerboard kernel can have)."""
+@pytest.mark.parametrize("case", ["int8-2d", "int8-3d-heatbath", "msc", "mixed"])
+def test_dependency_ordered_split_lattices_match_oracle(hip, case):
+    """Dependency-ordered launches where one lattice dominates the batch: it is cut into eight
+    bands, one per XCD, whose edge tiles use the device-scope fences, while small lattices run
+    whole on one XCD. Several launches (3 + 4 sweeps) and one long launch (25 sweeps) against
+    the oracle, moments against the oracle's magnetisation after every sweep."""
     from pyising_amd.batch import BatchScan
 
-    specs = [(hip.HipInt8, (2048, 2048), 1 / 2.269, "metropolis"),
-             (hip.HipMultiSpin, (4096, 4096), 1 / 2.269, "metropolis"),
-             (hip.HipInt8, (64, 64, 64), 0.2216, "heatbath")]
-    sims = [cls(sh, b, seed=31 + i, algorithm=al) for i, (cls, sh, b, al) in enumerate(specs)]
-    refs = [cls(sh, b, seed=31 + i, algorithm=al) for i, (cls, sh, b, al) in enumerate(specs)]
-    scan = BatchScan(sims, persistent=True, blocks_per_cu=1)
-    info = scan.info()
-    assert info["tiles_per_half_sweep"] > 2 * info["grid"]
-    scan.sweep(3)
-    scan.synchronize()
-    for sim, ref in zip(sims, refs, strict=True):
-        ref.sweep(3)
-        assert sim.observables_exact() == ref.observables_exact()
-        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    big = {"int8-2d": (hip.HipInt8, (512, 512), 1 / 2.269, "metropolis"),
+           "int8-3d-heatbath": (hip.HipInt8, (64, 64, 64), 0.2216, "heatbath"),
+           "msc": (hip.HipMultiSpin, (2048, 4096), 1 / 2.269, "metropolis"),
+           "mixed": (hip.HipInt8, (1024, 1024), 1 / 2.3, "metropolis")}[case]
+    specs = [(big[0], big[1], big[2], big[3], 900, O.random_lattice(big[1], 5))]
+    if case == "mixed":
+        specs += [s for s in _mixed(hip, seed0=300)[:8]]
+    sims = [_make(s) for s in specs]
+    scan = BatchScan(sims, persistent=True)
+    refs = [_oracle(s) for s in specs]
+    m_ref = [[] for _ in specs]
+    for calls in ((3, 4), (25,)):
+        for n in calls:
+            scan.sweep(n)
+            for i, r in enumerate(refs):
+                for _ in range(n):
+                    r.sweep(1)
+                    m_ref[i].append(r.observables_exact()[0])
+        scan.synchronize()
+        for sim, ref in zip(sims, refs, strict=True):
+            np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    for mom, ms in zip(scan.moments(), m_ref, strict=True):
+        m = np.array(ms, dtype=object)
+        assert mom.samples == len(ms) == 32
+        assert mom.sum_m == int(m.sum())
+        assert mom.sum_abs_m == int(abs(m).sum())
+        assert mom.sum_m2 == int((m * m).sum())
+        assert mom.sum_m4 == int((m**4).sum())
     scan.close()
 
 
@@ -144,7 +161,8 @@ def test_batch_moments_are_exact(hip, persistent):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("persistent,moments", [(False, True), (True, True), (False, False)])
+@pytest.mark.parametrize("persistent,moments", [(False, True), (True, True), (False, False),
+                                                (True, False)])
 def test_batch_graph_replays_continue_the_stream(hip, persistent, moments):
     from pyising_amd.batch import BatchScan
 
@@ -154,7 +172,7 @@ def test_batch_graph_replays_continue_the_stream(hip, persistent, moments):
     scan = BatchScan(sims, persistent=persistent, moments=moments)
     if not scan.capture(2):
         scan.close()
-        pytest.skip("the runtime cannot capture a cooperative launch into a graph")
+        pytest.skip("the runtime cannot capture the batch's launches into a graph")
     scan.replay(3)
     scan.sweep(1)  # a direct launch after replays continues the same counter
     scan.synchronize()
@@ -266,11 +284,13 @@ def test_reserved_batch_limits_across_assigns(hip):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("persistent", [False, True])
 @pytest.mark.parametrize("k", [2, 4, 8])
-def test_tiles_per_workgroup_change_nothing(hip, k):
-    """k tiles per workgroup (the lattices-per-workgroup schedule): lattices and moment sums
-    equal the one-tile form and the oracle, also with k changed between calls and under graph
-    replay; bad k is rejected."""
+def test_tiles_per_workgroup_change_nothing(hip, k, persistent):
+    """k tiles per workgroup (the lattices-per-workgroup schedule; in the dependency-ordered
+    form k consecutive tiles of one lattice share one wait): lattices and moment sums equal the
+    one-tile form and the oracle, also with k changed between calls and under graph replay; bad
+    k is rejected."""
     from pyising_amd import _lib
     from pyising_amd.batch import BatchScan
 
@@ -278,7 +298,7 @@ def test_tiles_per_workgroup_change_nothing(hip, k):
     sims = [_make(s) for s in specs]
     base = [_make(s) for s in specs]
     refs = [_make(s) for s in specs]
-    scan = BatchScan(sims, tiles_per_workgroup=k)
+    scan = BatchScan(sims, tiles_per_workgroup=k, persistent=persistent)
     ref_scan = BatchScan(base)
     scan.sweep(3)
     scan.set_tiles_per_workgroup(1)
