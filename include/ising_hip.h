@@ -113,6 +113,23 @@ int ising_int8_half(int8_t* d_own, const int8_t* d_other, long long rows, int lh
 int ising_int8_sweeps(int8_t* d_c0, int8_t* d_c1, long long rows, int lh, int ly, int lz, int ndim,
                       int heatbath, const uint32_t* table, uint32_t t0, long long n_sweeps,
                       uint32_t k0, uint32_t k1, void* stream);
+/* Resident sweeps of a 2D lattice (rows x 2 lh): the same n_sweeps sweeps as ising_int8_sweeps
+ * (bit-identical lattices) in one cooperative launch, each workgroup keeping a band of rows of
+ * both colours plus halo rows in LDS and exchanging edge rows with its neighbours every few
+ * half sweeps; for lattices too small to fill the GPU between per-half-sweep launches. Same
+ * entry point the reference's sweep (base.py:85-90) would bind.
+ * ising_int8_resident_bytes: the workspace bytes for `bands` bands (0: the library's choice),
+ * 0 when the form does not apply (lh % 16, rows, LDS, workgroup placement). The workspace is
+ * zeroed before its first use and kept between calls with the same `bands`; *epoch (0 after
+ * zeroing) is advanced by each call. Calls on one workspace must not overlap. */
+int ising_int8_resident_bytes(long long rows, int lh, int bands, size_t* bytes);
+int ising_int8_sweeps_resident(int8_t* d_c0, int8_t* d_c1, long long rows, int lh, int heatbath,
+                               const uint32_t* table, uint32_t t0, long long n_sweeps, uint32_t k0,
+                               uint32_t k1, int bands, uint32_t* d_ws, uint32_t* epoch,
+                               void* stream);
+/* Failure bits of the resident launches since the workspace was zeroed (synchronizes the
+ * stream): 1 = a bounded wait for a neighbour band gave up; the lattice is then invalid. */
+int ising_int8_resident_status(const uint32_t* d_ws, int* failed, void* stream);
 /* Exact (M, E) into out2[0..1] (device int64); d_scratch2 = ISING_OBS_SCRATCH_BYTES of device
  * memory (any contents).
  * observables_int8, src/pyising/backends/observables_torch.py:46-69. */

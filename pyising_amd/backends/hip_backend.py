@@ -26,6 +26,9 @@ _ALL_ONES = 0xFFFFFFFFFFFFFFFF
 
 #: msc kernel modes of ising_msc_sweeps_ex / ising_msc_sweeps_sync
 MSC_DEFAULT, MSC_COMPACT, MSC_LOCKSTEP, MSC_EAGER, MSC_BY_SIZE, MSC_FUSED = -1, 0, 1, 2, 3, 4
+#: int8 sweep forms: by size (resident sweeps for the 2D lattices they apply to), one launch
+#: per half sweep, or resident sweeps (one cooperative launch per call, bands kept in LDS)
+INT8_DEFAULT, INT8_HALVES, INT8_RESIDENT = -1, 0, 1
 #: ISING_OBS_SCRATCH_BYTES of ising_hip.h: partial sums of the observable reductions
 OBS_SCRATCH_BYTES = 16384
 
@@ -158,6 +161,13 @@ class HipInt8(_HipBase):
     """int8 compressed checkerboard; V sites per lane, one Philox call per 4 sites."""
 
     variant = "int8"
+    #: sweep form (INT8_DEFAULT, INT8_HALVES, INT8_RESIDENT) and, for resident sweeps, the
+    #: number of bands (0: the library's choice); set before the first sweep
+    mode = INT8_DEFAULT
+    resident_bands = 0
+    #: by size: resident sweeps from this many sweeps per call, for lattices up to this size
+    RESIDENT_MIN_SWEEPS = 2
+    RESIDENT_MAX_SPINS = 2048 * 2048
 
     def _setup(self) -> None:
         self._hip_setup()
@@ -166,6 +176,62 @@ class HipInt8(_HipBase):
         nbytes = self.rows * self.lh
         self.c = [self._alloc(nbytes), self._alloc(nbytes)]
         self._table_arr = np.ascontiguousarray(self.table, dtype=np.uint32)
+        self._rs = None  # workspace of the resident sweeps, made at their first use
+        self._rs_bytes = None
+        self._rs_epoch = C.c_uint32(0)
+        self._rs_unchecked = False
+
+    def resident_applies(self) -> bool:
+        """Whether resident sweeps can run this lattice on this device (2D, lh % 16 == 0,
+        bands and LDS within limits, a known workgroup placement)."""
+        if self.ndim != 2:
+            return False
+        if self._rs_bytes is None:
+            n = C.c_size_t()
+            _lib.check(self._L.ising_int8_resident_bytes(self.rows, self.lh,
+                                                         int(self.resident_bands), C.byref(n)),
+                       "int8_resident_bytes")
+            self._rs_bytes = n.value
+        return self._rs_bytes > 0
+
+    def _resident(self, n: int) -> bool:
+        if self.mode == INT8_HALVES:
+            return False
+        if self.mode == INT8_RESIDENT:
+            if not self.resident_applies():
+                raise ValueError("resident int8 sweeps do not apply to this lattice here")
+            return True
+        return (n >= self.RESIDENT_MIN_SWEEPS and self.n_spins <= self.RESIDENT_MAX_SPINS
+                and self.resident_applies())
+
+    def check_resident(self) -> None:
+        """Raise if a resident launch since the last check failed (a bounded wait for a
+        neighbour band gave up); the lattice is then invalid and the workspace is zeroed again.
+        Synchronous."""
+        if not self._rs_unchecked or self._rs is None:
+            return
+        failed = C.c_int()
+        _lib.check(self._L.ising_int8_resident_status(self._rs.ptr, C.byref(failed),
+                                                      self.stream_ptr), "resident_status")
+        self._rs_unchecked = False
+        if failed.value:
+            _lib.check(self._L.ising_fill_u64(self._rs.ptr, self._rs.nbytes // 8, 0,
+                                              self.stream_ptr), "fill")
+            self._rs_epoch.value = 0
+            raise _lib.HipError(f"resident int8 sweep failed (status {failed.value}): the "
+                                "lattice is invalid; sweep with mode=INT8_HALVES instead")
+
+    def synchronize(self) -> None:
+        super().synchronize()
+        self.check_resident()
+
+    def close(self) -> None:
+        try:
+            super().close()  # waits, and reports a resident failure not yet checked
+        finally:
+            if getattr(self, "_rs", None) is not None:
+                self._rs.free()
+                self._rs = None
 
     def _upload_table(self) -> None:
         self._table_arr = np.ascontiguousarray(self.table, dtype=np.uint32)
@@ -192,6 +258,7 @@ class HipInt8(_HipBase):
 
     def get_lattice(self) -> np.ndarray:
         self._after_batch()
+        self.check_resident()
         full = np.empty(self.shape, dtype=np.int8)
         buf, tmp = self._full_buffer(full.nbytes)
         _lib.check(self._L.ising_merge_int8(self.c[0].ptr, self.c[1].ptr, buf.ptr, self.rows,
@@ -215,10 +282,23 @@ class HipInt8(_HipBase):
         if n <= 0:
             return
         self._own_sweep_allowed()
-        _lib.check(self._L.ising_int8_sweeps(
-            self.c[0].ptr, self.c[1].ptr, self.rows, self.lh, self.ly, self.lz, self.ndim,
-            int(self.algorithm == "heatbath"), self._table_arr.ctypes.data,
-            self.t & 0xFFFFFFFF, n, self.k0, self.k1, self.stream_ptr), "int8_sweeps")
+        if self._resident(n):
+            if self._rs is None:
+                self._rs = _lib.DeviceBuffer(self._rs_bytes)
+                _lib.check(self._L.ising_fill_u64(self._rs.ptr, self._rs_bytes // 8, 0,
+                                                  self.stream_ptr), "fill")
+                self._rs_epoch.value = 0
+            _lib.check(self._L.ising_int8_sweeps_resident(
+                self.c[0].ptr, self.c[1].ptr, self.rows, self.lh,
+                int(self.algorithm == "heatbath"), self._table_arr.ctypes.data,
+                self.t & 0xFFFFFFFF, n, self.k0, self.k1, int(self.resident_bands), self._rs.ptr,
+                C.byref(self._rs_epoch), self.stream_ptr), "int8_sweeps_resident")
+            self._rs_unchecked = True
+        else:
+            _lib.check(self._L.ising_int8_sweeps(
+                self.c[0].ptr, self.c[1].ptr, self.rows, self.lh, self.ly, self.lz, self.ndim,
+                int(self.algorithm == "heatbath"), self._table_arr.ctypes.data,
+                self.t & 0xFFFFFFFF, n, self.k0, self.k1, self.stream_ptr), "int8_sweeps")
         self.t += n
 
     def _obs_into(self, out_ptr) -> None:
@@ -228,6 +308,7 @@ class HipInt8(_HipBase):
             self._obs_scratch.ptr, out_ptr, self.stream_ptr), "int8_obs")
 
     def observables_exact(self) -> tuple[int, int]:
+        self.check_resident()
         self._obs_into(self._obs_out.ptr)
         return self._read_obs()
 

@@ -123,49 +123,13 @@ __device__ __forceinline__ void int8_fill_table(u32* thr_lds, const Int8Table& t
     }
 }
 
-// Update the V consecutive colour sites of chunk ci (one row segment) of colour `color`.
-// Per 4-site word: one Philox call (scalar-uniform rounds: counter words 1-3 are sweep, colour
-// and 0); neighbour counts as bytes 4k from (s ^ n) & 0x04040404 (Metropolis: anti-aligned;
-// spins are 0x01 / 0xFF, so bit 2 differs exactly for opposite spins) or n & 0x04040404 (heat
-// bath: -1 neighbours); each byte addresses its site's threshold in the LDS table; the four
-// borrows of u - threshold accumulate into a 4-bit vector, spread to bytes by one multiply
-// ((b * 0x204081) & 0x01010101 moves bit b to byte b). Metropolis adds the forced flips
-// (count >= d) by a bytewise compare and flips with s ^ 0xFE; heat bath writes 0x01 or 0xFF.
-// Returns the number of -1 spins written (for magnetisation sums when COUNT).
-// COH: the other colour and the own sites are read with loads that miss in the L1.
-template <int NDIM, int HB, int V, bool COUNT, bool COH = false>
-__device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
-                                                 const int8_t* __restrict__ other, const Int8Geom& g,
-                                                 const u32* thr_lds, u32 t, u32 color, u32 k0,
-                                                 u32 k1, u32 ci)
+// Horizontal neighbours of V consecutive colour sites j0 .. j0 + V - 1, from the other colour's
+// vector C at the same sites and the edge byte e beyond them: p = 1 takes site j + 1 (e = site
+// j0 + V), p = 0 takes site j - 1 (e = site j0 - 1).
+template <int V>
+__device__ __forceinline__ Vec<V> int8_horizontal(const Vec<V>& C, u32 e, u32 p)
 {
     constexpr int NWd = V / 4;
-    constexpr u32 K4 = 0x04040404u;
-    constexpr u32 FORCE = (128u - 4u * NDIM) * LOW_BITS;  // byte >= 4d sets its bit 7
-    const u32 row = fast_div(ci, g.div_chunks);
-    const u32 j0 = (ci - row * g.chunks_row) * V;
-    const u32 z = fast_div(row, g.div_ly);
-    const u32 y = row - z * g.ly;
-    const u32 p = (y + z + color) & 1u;
-    const u64 rbase = (u64)row * g.lh;
-    const u64 n0 = rbase + j0;
-
-    const Vec<V> S = load_vec_at<V, COH>(own, n0);
-    const Vec<V> C = load_vec_at<V, COH>(other, n0);
-    const u32 yu = (y + 1 == g.ly) ? 0u : y + 1;
-    const u32 yd = (y == 0) ? g.ly - 1 : y - 1;
-    const Vec<V> U = load_vec_at<V, COH>(other, ((u64)z * g.ly + yu) * g.lh + j0);
-    const Vec<V> D = load_vec_at<V, COH>(other, ((u64)z * g.ly + yd) * g.lh + j0);
-    Vec<V> ZU, ZD;
-    if constexpr (NDIM == 3) {
-        const u32 zu = (z + 1 == g.lz) ? 0u : z + 1;
-        const u32 zd = (z == 0) ? g.lz - 1 : z - 1;
-        ZU = load_vec_at<V, COH>(other, ((u64)zu * g.ly + y) * g.lh + j0);
-        ZD = load_vec_at<V, COH>(other, ((u64)zd * g.ly + y) * g.lh + j0);
-    }
-    // horizontal neighbour: j + 1 when p == 1, j - 1 when p == 0 (one edge byte from memory)
-    const u32 je = p ? ((j0 + V == g.lh) ? 0u : j0 + V) : ((j0 == 0) ? g.lh - 1 : j0 - 1);
-    const u32 e = load_byte_at<COH>(other, rbase + je);
     Vec<V> H;
     if (p) {
 #pragma unroll
@@ -180,10 +144,32 @@ __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
             H.w[i] = __builtin_amdgcn_alignbit(C.w[i], prv, 24u);
         }
     }
+    return H;
+}
 
+// The new spins of V consecutive colour sites of colour `color` whose first site has colour
+// index n0 (own spins S; the other colour at the same sites C, beside them H, in the rows
+// either side U and D and, in 3D, in the planes either side ZU and ZD).
+// Per 4-site word: one Philox call (scalar-uniform rounds: counter words 1-3 are sweep, colour
+// and 0); neighbour counts as bytes 4k from (s ^ n) & 0x04040404 (Metropolis: anti-aligned;
+// spins are 0x01 / 0xFF, so bit 2 differs exactly for opposite spins) or n & 0x04040404 (heat
+// bath: -1 neighbours); each byte addresses its site's threshold in the LDS table; the four
+// borrows of u - threshold accumulate into a 4-bit vector, spread to bytes by one multiply
+// ((b * 0x204081) & 0x01010101 moves bit b to byte b). Metropolis adds the forced flips
+// (count >= d) by a bytewise compare and flips with s ^ 0xFE; heat bath writes 0x01 or 0xFF.
+// minus (when COUNT): the number of -1 spins written.
+template <int NDIM, int HB, int V, bool COUNT>
+__device__ __forceinline__ Vec<V> int8_decide(const Vec<V>& S, const Vec<V>& C, const Vec<V>& H,
+                                              const Vec<V>& U, const Vec<V>& D, const Vec<V>& ZU,
+                                              const Vec<V>& ZD, u64 n0, const u32* thr_lds,
+                                              u32 t, u32 color, u32 k0, u32 k1, u32& minus)
+{
+    constexpr int NWd = V / 4;
+    constexpr u32 K4 = 0x04040404u;
+    constexpr u32 FORCE = (128u - 4u * NDIM) * LOW_BITS;  // byte >= 4d sets its bit 7
     const uint8_t* thr_bytes = reinterpret_cast<const uint8_t*>(thr_lds);
     Vec<V> O;
-    u32 minus = 0;
+    minus = 0;
 #pragma unroll
     for (int i = 0; i < NWd; ++i) {
         const u32 q = (u32)((n0 >> 2) + (u64)i);  // quad index, truncated to 32 bits
@@ -212,6 +198,47 @@ __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
         O.w[i] = HB ? ~m : (S.w[i] ^ m);
         if constexpr (COUNT) minus += __popc(O.w[i] & 0x80808080u);
     }
+    return O;
+}
+
+// Update the V consecutive colour sites of chunk ci (one row segment) of colour `color` in
+// global memory (int8_decide on the loaded neighbours). Returns the number of -1 spins written
+// (for magnetisation sums when COUNT).
+// COH: the other colour and the own sites are read with loads that miss in the L1.
+template <int NDIM, int HB, int V, bool COUNT, bool COH = false>
+__device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
+                                                 const int8_t* __restrict__ other, const Int8Geom& g,
+                                                 const u32* thr_lds, u32 t, u32 color, u32 k0,
+                                                 u32 k1, u32 ci)
+{
+    const u32 row = fast_div(ci, g.div_chunks);
+    const u32 j0 = (ci - row * g.chunks_row) * V;
+    const u32 z = fast_div(row, g.div_ly);
+    const u32 y = row - z * g.ly;
+    const u32 p = (y + z + color) & 1u;
+    const u64 rbase = (u64)row * g.lh;
+    const u64 n0 = rbase + j0;
+
+    const Vec<V> S = load_vec_at<V, COH>(own, n0);
+    const Vec<V> C = load_vec_at<V, COH>(other, n0);
+    const u32 yu = (y + 1 == g.ly) ? 0u : y + 1;
+    const u32 yd = (y == 0) ? g.ly - 1 : y - 1;
+    const Vec<V> U = load_vec_at<V, COH>(other, ((u64)z * g.ly + yu) * g.lh + j0);
+    const Vec<V> D = load_vec_at<V, COH>(other, ((u64)z * g.ly + yd) * g.lh + j0);
+    Vec<V> ZU, ZD;  // 3D only
+    if constexpr (NDIM == 3) {
+        const u32 zu = (z + 1 == g.lz) ? 0u : z + 1;
+        const u32 zd = (z == 0) ? g.lz - 1 : z - 1;
+        ZU = load_vec_at<V, COH>(other, ((u64)zu * g.ly + y) * g.lh + j0);
+        ZD = load_vec_at<V, COH>(other, ((u64)zd * g.ly + y) * g.lh + j0);
+    }
+    // horizontal neighbour: j + 1 when p == 1, j - 1 when p == 0 (one edge byte from memory)
+    const u32 je = p ? ((j0 + V == g.lh) ? 0u : j0 + V) : ((j0 == 0) ? g.lh - 1 : j0 - 1);
+    const u32 e = load_byte_at<COH>(other, rbase + je);
+    const Vec<V> H = int8_horizontal<V>(C, e, p);
+    u32 minus = 0;
+    const Vec<V> O = int8_decide<NDIM, HB, V, COUNT>(S, C, H, U, D, ZU, ZD, n0, thr_lds, t, color,
+                                                     k0, k1, minus);
     store_vec<V>(own + n0, O);
     return minus;
 }
