@@ -259,7 +259,7 @@ int ising_batch_info(void* handle, int* grid, int* total_tiles, int* max_blocks_
 /* Form of a sweep call: 0 (default) = one launch per half sweep (+ one accumulation launch per
  * sweep while moments are on); 1 = dependency-ordered launches of up to 512 sweeps each (per-tile
  * completion counters; each lattice runs on one XCD, or in eight bands, one per XCD), followed
- * by a one-workgroup check of the counters. 1 needs a batch made by ising_batch_create and a
+ * by a check of the counters. 1 needs a batch made by ising_batch_create and a
  * device that places workgroups round-robin over its XCDs or all on one (ISING_EINVAL
  * otherwise). */
 int ising_batch_set_persistent(void* handle, int persistent);

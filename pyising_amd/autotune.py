@@ -22,7 +22,7 @@ from dataclasses import asdict, dataclass, field
 from pathlib import Path
 
 GRAPH_CHOICES = (0, 10, 50)
-TILES_PER_WG_CHOICES = (1, 2, 4)  # tiles (lattices or parts of one) per workgroup, either form
+TILES_PER_WG_CHOICES = (1, 2, 4, 8)  # tiles (lattices or parts of one) per workgroup, either form
 
 
 @dataclass

@@ -314,7 +314,7 @@ __global__ void batch_zero_u64(unsigned long long* p, u32 n)
 // which each XCD dispatches first: the oldest unfinished workgroup can always proceed. Waits
 // are bounded (BF_SPIN_LIMIT polls): a workgroup that gives up sets a failure bit and exits
 // without touching the lattice, later waiters see the bit and exit at once, the kernel ends
-// normally, and batch_fused_end checks every counter.
+// normally, and batch_fused_check checks every counter.
 //
 // Memory. Inside one XCD the L2 is the coherence point: stores are acknowledged (s_waitcnt)
 // before the counter is published, every lattice load misses in the CU's L1 (cache policy sc1;
@@ -532,26 +532,29 @@ __global__ __launch_bounds__(256) void batch_fused(BatchFusedArgs f)
     }
 }
 
-// After a dependency-ordered launch of n sweeps (one workgroup): every tile of a lattice that
-// took part must have published its last half sweep, else a tile went missing (failure bit 2);
-// then the batch's sweep counter advances by n.
-__global__ __launch_bounds__(256) void batch_fused_end(const BatchDesc* descs, const u32* tile_lat,
-                                                       u32 total_tiles, u32* sync,
-                                                       unsigned long long* sweeps_done, u32 n)
+// After a dependency-ordered launch of n sweeps: every tile of a lattice that took part must
+// have published its last half sweep, else a tile went missing (failure bit 2). One thread per
+// tile; batch_fused_advance then moves the batch's sweep counter on by n.
+__global__ __launch_bounds__(256) void batch_fused_check(const BatchDesc* descs, const u32* tile_lat,
+                                                         u32 total_tiles, u32* sync,
+                                                         const unsigned long long* sweeps_done, u32 n)
 {
+    const u32 t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= total_tiles) return;
     const u32 sw0 = (u32)sweeps_done[0];
-    int bad = 0;
-    for (u32 t = threadIdx.x; t < total_tiles; t += 256u) {
-        const BatchDesc& d = descs[tile_lat[t]];
-        if (sw0 >= d.limit) continue;
-        const u32 end = (d.limit - sw0 < n) ? d.limit : sw0 + n;
-        if (sync[BF_HDR + t] != 2u * end) bad = 1;
-    }
-    bad = __syncthreads_or(bad);
+    const BatchDesc& d = descs[tile_lat[t]];
+    if (sw0 >= d.limit) return;
+    const u32 end = (d.limit - sw0 < n) ? d.limit : sw0 + n;
+    if (sync[BF_HDR + t] != 2u * end)
+        __hip_atomic_fetch_or(sync, BF_FAIL_COUNTERS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void batch_fused_advance(unsigned long long* sweeps_done, u32 n)
+{
     if (threadIdx.x == 0) {
-        if (bad) sync[0] |= BF_FAIL_COUNTERS;
-        sweeps_done[0] = sw0 + n;
-        sweeps_done[1] = sw0 + n - 1u;
+        const unsigned long long sw = sweeps_done[0] + n;
+        sweeps_done[0] = sw;
+        sweeps_done[1] = sw - 1u;
     }
 }
 
@@ -1050,8 +1053,10 @@ static int batch_launch_fused(BatchHandle* h, u32 n_sweeps, hipStream_t st)
             ISING_HIP(hipEventRecord(h->f_join, h->f_side));
             ISING_HIP(hipStreamWaitEvent(st, h->f_join, 0));
         }
-        hipLaunchKernelGGL(batch_fused_end, dim3(1), dim3(256), 0, st, (const BatchDesc*)h->d_descs,
-                           (const u32*)h->d_tile_lat, h->total_tiles, h->d_fsync, h->d_done, k);
+        hipLaunchKernelGGL(batch_fused_check, dim3((h->total_tiles + 255u) / 256u), dim3(256), 0, st,
+                           (const BatchDesc*)h->d_descs, (const u32*)h->d_tile_lat, h->total_tiles,
+                           h->d_fsync, (const unsigned long long*)h->d_done, k);
+        hipLaunchKernelGGL(batch_fused_advance, dim3(1), dim3(64), 0, st, h->d_done, k);
         done += k;
     }
     ISING_LAUNCHED("batch_fused");

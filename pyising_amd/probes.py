@@ -119,33 +119,43 @@ def _run_msc(L: int, sweeps: int, mode: int, k: int) -> dict:
             "algorithmic_bytes_per_launch": int(0.375 * L * L / 2)}
 
 
-def _run_int8(L: int, sweeps: int, dim: int, algorithm: str) -> dict:
+def _run_int8(L: int, sweeps: int, dim: int, algorithm: str, mode: int = -1) -> dict:
     from .backends.hip_backend import HipInt8
     from .lattice import BETA_C_2D, BETA_C_3D
 
     beta = BETA_C_2D if dim == 2 else BETA_C_3D
     sim = HipInt8((L,) * dim, beta, seed=1, algorithm=algorithm)
+    sim.mode = mode  # -1 by size, 0 one launch per half sweep, 1 resident sweeps
     sim.sweep(sweeps)
+    sim.synchronize()
     sim.close()
     return {"probe": "int8", "L": L, "dim": dim, "sweeps": sweeps, "launches": 2 * sweeps,
             "algorithmic_bytes_per_launch": int(3.0 * L**dim / 2)}
 
 
-def _run_batch(sweeps: int, msc_tile: str, moments: bool) -> dict:
+def _run_batch(sweeps: int, msc_tile: str, moments: bool, persistent: bool = False,
+               k: int = 1) -> dict:
     """The Binder-scan batch (96 int8 lattices L = 64..2048, 32 msc lattices L = 1024, 4096,
-    16 temperatures each), one launch per half sweep, fixed tiles, no graph."""
+    16 temperatures each), fixed tiles (msc_tile "mixed": lockstep for L = 1024, queue
+    compaction for L = 4096), no graph; persistent: dependency-ordered launches; k tiles per
+    workgroup."""
     from .batch import BatchScan, temperature_scan_lattices
 
     sims = temperature_scan_lattices()
-    tiles = {(1024, 1024): msc_tile, (4096, 4096): msc_tile}
-    scan = BatchScan(sims, msc_tiles=tiles, moments=moments)
+    if msc_tile == "mixed":
+        tiles = {(1024, 1024): "lockstep", (4096, 4096): "queue"}
+    else:
+        tiles = {(1024, 1024): msc_tile, (4096, 4096): msc_tile}
+    scan = BatchScan(sims, msc_tiles=tiles, moments=moments, persistent=persistent,
+                     tiles_per_workgroup=k)
     scan.sweep(sweeps)
     scan.synchronize()
     scan.close()
     for sim in sims:
         sim.close()
     return {"probe": "batch", "lattices": len(sims), "spins": int(sum(s.n_spins for s in sims)),
-            "sweeps": sweeps, "msc_tile": msc_tile, "moments": moments}
+            "sweeps": sweeps, "msc_tile": msc_tile, "moments": moments,
+            "persistent": persistent, "tiles_per_workgroup": k}
 
 
 def _run_serve(requests: int, rate: float) -> dict:
@@ -175,6 +185,7 @@ def main(argv=None) -> int:
     i.add_argument("--dim", type=int, default=2)
     i.add_argument("--sweeps", type=int, default=10)
     i.add_argument("--algorithm", default="metropolis")
+    i.add_argument("--mode", type=int, default=-1, choices=(-1, 0, 1))
     ph = sub.add_parser("philox")
     ph.add_argument("--calls", type=int, default=512)
     ph.add_argument("--reps", type=int, default=3)
@@ -182,8 +193,10 @@ def main(argv=None) -> int:
     sub.add_parser("digest")
     b = sub.add_parser("batch")
     b.add_argument("--sweeps", type=int, default=50)
-    b.add_argument("--msc-tile", default="lockstep", choices=("lockstep", "queue"))
+    b.add_argument("--msc-tile", default="lockstep", choices=("lockstep", "queue", "mixed"))
     b.add_argument("--no-moments", action="store_true")
+    b.add_argument("--persistent", action="store_true")
+    b.add_argument("--k", type=int, default=1, choices=(1, 2, 4, 8))
     sv = sub.add_parser("serve")
     sv.add_argument("--requests", type=int, default=300)
     sv.add_argument("--rate", type=float, default=200.0)
@@ -196,9 +209,9 @@ def main(argv=None) -> int:
     elif a.cmd == "msc":
         out = _run_msc(a.L, a.sweeps, a.mode, a.k)
     elif a.cmd == "int8":
-        out = _run_int8(a.L, a.sweeps, a.dim, a.algorithm)
+        out = _run_int8(a.L, a.sweeps, a.dim, a.algorithm, a.mode)
     elif a.cmd == "batch":
-        out = _run_batch(a.sweeps, a.msc_tile, not a.no_moments)
+        out = _run_batch(a.sweeps, a.msc_tile, not a.no_moments, a.persistent, a.k)
     elif a.cmd == "serve":
         out = _run_serve(a.requests, a.rate)
     elif a.cmd == "digest":
