@@ -186,6 +186,24 @@ int ising_msc_sweeps_sync(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw
                           long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4,
                           uint32_t thr8, int mode, int k, uint32_t* d_sync, uint32_t* epoch,
                           unsigned long long* d_stats, void* stream);
+/* Resident sweeps of a 2D packed lattice (rows x 128 nw, nw even): the same n_sweeps sweeps as
+ * the per-half-sweep kernels (bit-identical words) in one cooperative launch, each workgroup
+ * keeping a band of rows of both packed colours plus halo rows in LDS and exchanging edge rows
+ * with its neighbours every few half sweeps; for lattices up to 4096^2, whose half sweeps are
+ * too short to amortize a launch (8192^2 fits but runs no faster). The lazy comparison is the
+ * lockstep one. Same
+ * entry point the reference's sweep (base.py:85-90) would bind.
+ * ising_msc_resident_bytes: the workspace bytes for `bands` bands (0: the library's choice),
+ * 0 when the form does not apply (nw odd, rows, LDS, workgroup placement); workspace and *epoch
+ * as for ising_int8_sweeps_resident. */
+int ising_msc_resident_bytes(long long rows, int nw, int bands, size_t* bytes);
+int ising_msc_sweeps_resident(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw, uint32_t t0,
+                              long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4,
+                              uint32_t thr8, int bands, uint32_t* d_ws, uint32_t* epoch,
+                              void* stream);
+/* Failure bits of the resident launches since the workspace was zeroed (synchronizes the
+ * stream): 1 = a bounded wait for a neighbour band gave up; the lattice is then invalid. */
+int ising_msc_resident_status(const uint32_t* d_ws, int* failed, void* stream);
 /* Bytes of the sync workspace of a rows x (128 nw) lattice (host only). No reference
  * counterpart. */
 int ising_msc_sync_bytes(long long rows, int nw, size_t* bytes);

@@ -26,6 +26,9 @@ _ALL_ONES = 0xFFFFFFFFFFFFFFFF
 
 #: msc kernel modes of ising_msc_sweeps_ex / ising_msc_sweeps_sync
 MSC_DEFAULT, MSC_COMPACT, MSC_LOCKSTEP, MSC_EAGER, MSC_BY_SIZE, MSC_FUSED = -1, 0, 1, 2, 3, 4
+#: msc: resident sweeps (one cooperative launch per call, bands kept in LDS), a mode of the
+#: Python class only (the others are the C library's)
+MSC_RESIDENT = 5
 #: int8 sweep forms: by size (resident sweeps for the 2D lattices they apply to), one launch
 #: per half sweep, or resident sweeps (one cooperative launch per call, bands kept in LDS)
 INT8_DEFAULT, INT8_HALVES, INT8_RESIDENT = -1, 0, 1
@@ -166,6 +169,7 @@ class HipInt8(_HipBase):
     mode = INT8_DEFAULT
     resident_bands = 0
     #: by size: resident sweeps from this many sweeps per call, for lattices up to this size
+    #: (resident sweeps need rows of at most 64 KiB of LDS per band: L <= 2048 here)
     RESIDENT_MIN_SWEEPS = 2
     RESIDENT_MAX_SPINS = 2048 * 2048
 
@@ -327,6 +331,14 @@ class HipMultiSpin(_HipBase):
     #: queue compaction above
     mode = MSC_DEFAULT
     tile_k = 42
+    #: resident sweeps: the number of bands (0: the library's choice); by size (MSC_DEFAULT or
+    #: MSC_BY_SIZE) they run calls of at least RESIDENT_MIN_SWEEPS sweeps on lattices up to
+    #: RESIDENT_MAX_SPINS spins
+    resident_bands = 0
+    RESIDENT_MIN_SWEEPS = 2
+    #: measured, flips/ns best per-half-sweep kernel / resident: 1024^2 113 / 176, 2048^2
+    #: 429 / 672, 4096^2 1514 / 2036, 8192^2 3722 / 3683
+    RESIDENT_MAX_SPINS = 4096 * 4096
 
     def _setup(self) -> None:
         self._hip_setup()
@@ -337,6 +349,48 @@ class HipMultiSpin(_HipBase):
         self._sync = None  # workspace of the fused sweeps, made at the first sweep
         self._epoch = C.c_uint32(0)
         self._sync_unchecked = False
+        self._rs = None  # workspace of the resident sweeps, made at their first use
+        self._rs_bytes = None
+        self._rs_epoch = C.c_uint32(0)
+        self._rs_unchecked = False
+
+    def resident_applies(self) -> bool:
+        """Whether resident sweeps can run this lattice on this device (an even word count per
+        row, bands and LDS within limits, a known workgroup placement)."""
+        if self._rs_bytes is None:
+            n = C.c_size_t()
+            _lib.check(self._L.ising_msc_resident_bytes(self.rows, self.nw,
+                                                        int(self.resident_bands), C.byref(n)),
+                       "msc_resident_bytes")
+            self._rs_bytes = n.value
+        return self._rs_bytes > 0
+
+    def _resident(self, n: int) -> bool:
+        if self.stats_buf is not None:  # the Philox statistics come from the other kernels
+            return False
+        if self.mode == MSC_RESIDENT:
+            if not self.resident_applies():
+                raise ValueError("resident msc sweeps do not apply to this lattice here")
+            return True
+        return (self.mode in (MSC_DEFAULT, MSC_BY_SIZE) and n >= self.RESIDENT_MIN_SWEEPS
+                and self.n_spins <= self.RESIDENT_MAX_SPINS and self.resident_applies())
+
+    def check_resident(self) -> None:
+        """Raise if a resident launch since the last check failed (a bounded wait for a
+        neighbour band gave up); the lattice is then invalid and the workspace is zeroed again.
+        Synchronous."""
+        if not self._rs_unchecked or self._rs is None:
+            return
+        failed = C.c_int()
+        _lib.check(self._L.ising_msc_resident_status(self._rs.ptr, C.byref(failed),
+                                                     self.stream_ptr), "resident_status")
+        self._rs_unchecked = False
+        if failed.value:
+            _lib.check(self._L.ising_fill_u64(self._rs.ptr, self._rs.nbytes // 8, 0,
+                                              self.stream_ptr), "fill")
+            self._rs_epoch.value = 0
+            raise _lib.HipError(f"resident msc sweep failed (status {failed.value}): the "
+                                "lattice is invalid; sweep with mode=MSC_COMPACT instead")
 
     def _sync_buffer(self):
         """The fused sweeps' workspace (per-tile completion counters), zeroed once; its counter
@@ -371,13 +425,17 @@ class HipMultiSpin(_HipBase):
     def synchronize(self) -> None:
         super().synchronize()
         self.check_sync()
+        self.check_resident()
 
     def close(self) -> None:
         try:
-            super().close()  # waits, and reports a fused failure not yet checked
+            super().close()  # waits, and reports a fused or resident failure not yet checked
         finally:
             if self._sync is not None:
                 self._sync.free()
+            if getattr(self, "_rs", None) is not None:
+                self._rs.free()
+                self._rs = None
 
     def _split(self, buf, tmp: bool) -> None:
         self._after_batch()
@@ -408,6 +466,7 @@ class HipMultiSpin(_HipBase):
         if tmp:
             buf.free()
         self.check_sync()
+        self.check_resident()
         return full
 
     def kernel(self) -> dict:
@@ -445,6 +504,18 @@ class HipMultiSpin(_HipBase):
                 self.k0, self.k1, thr4, thr8, self.mode, self.tile_k, self._stats_ptr(),
                 self.stream_ptr), "msc_half")
             return
+        if self._resident(n):
+            if self._rs is None:
+                self._rs = _lib.DeviceBuffer(self._rs_bytes)
+                _lib.check(self._L.ising_fill_u64(self._rs.ptr, self._rs_bytes // 8, 0,
+                                                  self.stream_ptr), "fill")
+                self._rs_epoch.value = 0
+            _lib.check(self._L.ising_msc_sweeps_resident(
+                self.w[0].ptr, self.w[1].ptr, self.rows, self.nw, self.t & 0xFFFFFFFF, n,
+                self.k0, self.k1, thr4, thr8, int(self.resident_bands), self._rs.ptr,
+                C.byref(self._rs_epoch), self.stream_ptr), "msc_sweeps_resident")
+            self._rs_unchecked = True
+            return
         sync = self._sync_buffer()
         _lib.check(self._L.ising_msc_sweeps_sync(
             self.w[0].ptr, self.w[1].ptr, self.rows, self.nw, self.t & 0xFFFFFFFF, n,
@@ -469,6 +540,7 @@ class HipMultiSpin(_HipBase):
         self._obs_into(self._obs_out.ptr)
         out = self._read_obs()
         self.check_sync()
+        self.check_resident()
         return out
 
     def philox_stats(self, sweeps: int = 2) -> dict:

@@ -1,0 +1,134 @@
+// Resident multi-spin-coded sweeps of small 2D lattices: one launch holds every half sweep of a
+// call, each workgroup keeping a band of rows of both packed colours in LDS for the whole
+// launch (the band, round and exchange machinery is resident.hpp's).
+//
+// Why: at L <= 4096 a half sweep is one generation of short workgroups or less, and one launch
+// per half sweep spends most of its time in launch gaps, ramp and drain. Here the words never
+// leave the CUs between half sweeps. Measured (flips/ns, best per-half-sweep kernel against
+// resident, same box): 1024^2 113 / 176, 2048^2 429 / 672, 4096^2 1514 / 2036; at 8192^2 the
+// per-half-sweep compaction kernel is as fast (3722 / 3683). A band's half sweep holds few
+// words (L = 1024: at most 80 per workgroup), so each lockstep wave's chain of Philox calls,
+// not the VALU throughput, sets its time.
+//
+// Each lane owns one word of a row: own, the other colour at the same word, the other colour's
+// words in the rows either side and the horizontal neighbour word from LDS, the same
+// classification (msc_state) and lockstep lazy comparison (msc_run) as the per-half-sweep
+// lockstep kernel, with the global word index in the Philox counter (word, sweep, colour,
+// 1 + level pair): bit-identical words (src/pyising/backends/numpy_backend.py:103-124 is the
+// reference's half sweep).
+#include "msc_core.hpp"
+#include "resident.hpp"
+
+struct MscResidentArgs {
+    ResidentBands bands;  // row_bytes = 8 nw
+    u64* w0;
+    u64* w1;
+    u32 nw;
+    FastDiv div_nw;
+    u32 t0, k0, k1;
+    u64 codes;  // case code of every level pair (msc_level_codes)
+};
+
+__global__ __launch_bounds__(RS_THREADS) void msc_resident_kernel(MscResidentArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) int8_t rs_lds[];
+    __shared__ u32 go;
+    const u32 nw = a.nw, R = a.bands.R, T = a.bands.T, rows = a.bands.rows;
+    const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    rs_run(a.bands, rs_lds, (int8_t*)a.w0, (int8_t*)a.w1, &go,
+           [&](int8_t** lc, u32 hh, u32 ext, u32 first) {
+        MscArgs m;
+        m.color = hh & 1u;
+        m.t = a.t0 + (hh >> 1);
+        m.k0 = a.k0;
+        m.k1 = a.k1;
+        u64* own = reinterpret_cast<u64*>(lc[m.color]);
+        const u64* other = reinterpret_cast<const u64*>(lc[m.color ^ 1u]);
+        const u32 row_lo = T - ext, n = (R + 2u * ext) * nw;
+        // whole waves (the lazy loop ballots across the wave); lanes past the end idle
+        for (u32 base = wave * 64u; base < n; base += RS_THREADS) {
+            const u32 it = base + lane;
+            const bool valid = it < n;
+            const u32 itc = valid ? it : n - 1u;
+            const u32 rr = fast_div(itc, a.div_nw);
+            const u32 x = itc - rr * nw;
+            const u32 row = row_lo + rr;  // LDS row, 1 <= row <= R + 2T - 2
+            u32 y = first + row;
+            if (y >= rows) y -= rows;
+            const u32 p = (y + m.color) & 1u;
+            const u64* crow = other + (size_t)row * nw;
+            const u64 o = own[(size_t)row * nw + x];
+            const u64 c = crow[x];
+            const u32 xn = p ? ((x + 1u == nw) ? 0u : x + 1u) : ((x == 0u) ? nw - 1u : x - 1u);
+            const u64 nb = crow[xn];
+            const u64 hz = p ? ((c >> 1) | (nb << 63)) : ((c << 1) | (nb >> 63));
+            const u64 up = other[(size_t)(row + 1u) * nw + x];  // row y + 1
+            const u64 dn = other[(size_t)(row - 1u) * nw + x];  // row y - 1
+            MscState st = msc_state(o, c, hz, up, dn);
+            if (!valid) {
+                st.ul = 0u;
+                st.uh = 0u;
+            }
+            u32 calls = 0, slots = 0;
+            msc_run<true, false>(m, a.codes, y * nw + x, 0u, 16u, st, calls, slots);
+            if (valid) own[(size_t)row * nw + x] = (u64)st.pwl | ((u64)st.pwh << 32);
+        }
+        __syncthreads();
+    });
+}
+
+extern "C" {
+
+// Workspace bytes of the resident sweeps of a 2D packed lattice (rows x 128 nw) with `bands`
+// bands (0: the default choice), 0 when they do not apply here (nw odd, LDS, placement).
+int ising_msc_resident_bytes(long long rows, int nw, int bands, size_t* bytes)
+{
+    if (!bytes || rows < 1 || nw < 1 || bands < 0) return ising_fail(ISING_EINVAL, "msc_resident_bytes: bad argument");
+    ResidentPlan pl;
+    if (int rc = resident_plan(rows, 8ll * nw, bands, pl)) return rc;
+    *bytes = pl.ws_bytes;
+    return ISING_OK;
+}
+
+// n_sweeps sweeps of a 2D packed lattice in one cooperative launch (colour 0 then 1 per sweep,
+// Philox counters t0, t0 + 1, ...): bit-identical to the per-half-sweep msc kernels.
+// ws: a workspace of ising_msc_resident_bytes zeroed before first use and kept between calls;
+// *epoch: the rounds its counters hold (0 after zeroing), advanced by this call.
+int ising_msc_sweeps_resident(uint64_t* d_w0, uint64_t* d_w1, long long rows, int nw, uint32_t t0,
+                              long long n_sweeps, uint32_t k0, uint32_t k1, uint32_t thr4,
+                              uint32_t thr8, int bands, uint32_t* ws, uint32_t* epoch, void* stream)
+{
+    if (!d_w0 || !d_w1 || !ws || !epoch || n_sweeps < 0 || n_sweeps > (1ll << 30) || bands < 0 || nw < 1)
+        return ising_fail(ISING_EINVAL, "msc_sweeps_resident: bad argument");
+    if (n_sweeps == 0) return ISING_OK;
+    ResidentPlan pl;
+    if (int rc = resident_plan(rows, 8ll * nw, bands, pl)) return rc;
+    if (!pl.B) return ising_fail(ISING_EINVAL, "msc_sweeps_resident: not applicable to this lattice here");
+    MscResidentArgs a;
+    a.bands.rows = (u32)rows;
+    a.bands.row_bytes = 8u * (u32)nw;
+    a.bands.R = pl.R;
+    a.bands.T = pl.T;
+    a.bands.B = pl.B;
+    a.bands.one_l2 = pl.place == 0 ? 1u : 0u;
+    a.bands.ws = ws;
+    a.w0 = d_w0;
+    a.w1 = d_w1;
+    a.nw = (u32)nw;
+    a.div_nw = make_fastdiv((u32)nw);
+    a.k0 = k0;
+    a.k1 = k1;
+    a.codes = msc_level_codes(thr4, thr8);
+    return resident_launch((const void*)msc_resident_kernel, a, pl, t0, n_sweeps, epoch,
+                           as_stream(stream));
+}
+
+// Failure bits of the resident launches since the workspace was zeroed (synchronizes the
+// stream): 1 = a wait gave up (the lattice is then invalid).
+int ising_msc_resident_status(const uint32_t* ws, int* failed, void* stream)
+{
+    if (!ws || !failed) return ising_fail(ISING_EINVAL, "msc_resident_status: null pointer");
+    return resident_status(ws, failed, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,7 +1,8 @@
-"""Resident int8 sweeps (csrc/int8_resident.hip): a 2D lattice's half sweeps in one cooperative
-launch, bands of rows kept in LDS with redundant halo rows, edge rows exchanged between bands
-every few half sweeps. Every result is checked bit for bit against the oracle's C restatement
-(pinned to the NumPy oracle and the reference's own vectors by the CPU tests)."""
+"""Resident sweeps (csrc/resident.hpp, int8_resident.hip, msc_resident.hip): a 2D lattice's half
+sweeps in one cooperative launch, bands of rows kept in LDS with redundant halo rows, edge rows
+exchanged between bands every few half sweeps. Every result is checked bit for bit against the
+oracle's C restatement (pinned to the NumPy oracle and the reference's own vectors by the CPU
+tests)."""
 
 import numpy as np
 import pytest
@@ -110,3 +111,86 @@ def test_not_applicable(hip):
     assert not sim.resident_applies()
     sim.sweep(2)  # the default falls back to one launch per half sweep
     sim.close()
+
+
+# ---------------------------------------------------------------------------------- msc ----
+def _msc_pair(hip, shape, beta, seed, bands=0, mode=None):
+    lat = O.random_lattice(shape, seed + 17)
+    sim = hip.HipMultiSpin(shape, beta, seed=seed, init=lat)
+    sim.mode = hip.MSC_RESIDENT if mode is None else mode
+    sim.resident_bands = bands
+    ref = CO.CMsc(shape, beta, seed=seed, init=lat)
+    return sim, ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,beta", [
+    ((64, 256), 0.44),          # nw = 2
+    ((128, 512), 1 / 2.269),
+    ((1024, 1024), 1 / 2.269),
+    ((512, 2048), 0.5),         # nw = 16
+    ((2048, 2048), 1 / 2.269),
+    ((4096, 4096), 1 / 2.269),
+])
+def test_msc_resident_matches_oracle(hip, shape, beta):
+    """Calls of 1, 2, 3, 8 and 25 sweeps, each compared with the oracle after the call, plus M
+    and E."""
+    sim, ref = _msc_pair(hip, shape, beta, seed=4)
+    assert sim.resident_applies()
+    for n in (1, 2, 3, 8, 25):
+        sim.sweep(n)
+        ref.sweep(n)
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    assert sim.observables_exact() == tuple(ref.observables_exact())
+    sim.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bands", [8, 32, 128, 256])
+def test_msc_band_counts_change_nothing(hip, bands):
+    sim, ref = _msc_pair(hip, (1024, 1024), 0.44, seed=6, bands=bands)
+    if not sim.resident_applies():
+        sim.close()
+        pytest.skip(f"{bands} bands do not fit this lattice on this device")
+    sim.sweep(11)
+    ref.sweep(11)
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_msc_forms_interleave(hip):
+    """Resident calls and the other kernels on one lattice continue one stream; the default
+    picks resident sweeps for this size; the eager ablation never does."""
+    sim, ref = _msc_pair(hip, (2048, 2048), 1 / 2.269, seed=8, mode=hip.MSC_DEFAULT)
+    sim.sweep(3)
+    assert sim._rs is not None
+    sim.mode = hip.MSC_COMPACT
+    sim.sweep(2)
+    sim.mode = hip.MSC_LOCKSTEP
+    sim.sweep(1)
+    sim.mode = hip.MSC_RESIDENT
+    sim.sweep(4)
+    ref.sweep(10)
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    sim.close()
+    eager = hip.HipMultiSpinEager((1024, 1024), 0.44, seed=1)
+    eager.sweep(2)
+    assert eager._rs is None
+    eager.close()
+
+
+@pytest.mark.gpu
+def test_msc_not_applicable(hip):
+    sim = hip.HipMultiSpin((64, 128), 0.44, seed=1)  # nw = 1: rows of 8 bytes
+    assert not sim.resident_applies()
+    sim.mode = hip.MSC_RESIDENT
+    with pytest.raises(ValueError):
+        sim.sweep(2)
+    sim.mode = hip.MSC_DEFAULT
+    sim.sweep(2)
+    sim.close()
+    big = hip.HipMultiSpin((16384, 16384), 0.44, seed=1)  # the fused sweeps' size
+    big.sweep(2)
+    assert big._rs is None
+    big.close()
