@@ -106,13 +106,12 @@ __device__ __forceinline__ MscRowLoad msc_load_row(const MscArgs& a, u32 w0, u32
     L.up = msc_ld<COH>(a.other + (u64)yu * a.nw + x0 + lane);
     L.dn = msc_ld<COH>(a.other + (u64)yd * a.nw + x0 + lane);
     L.right = (y + a.color) & 1u;
-    if (L.right) {
-        const u32 xe = (x0 + 64u == a.nw) ? 0u : x0 + 64u;  // word x0 + 64, wrapped
-        L.e = (u32)msc_ld<COH>(crow + xe);
-    } else {
-        const u32 xe = (x0 == 0u) ? a.nw - 1u : x0 - 1u;  // word x0 - 1, wrapped
-        L.e = (u32)(msc_ld<COH>(crow + xe) >> 32);
-    }
+    // one load of the word beyond the edge (x0 + 64 or x0 - 1, wrapped), its half selected
+    // after the load, so that no wait for it sits between this and the next round's loads
+    const u32 xr = (x0 + 64u == a.nw) ? 0u : x0 + 64u;
+    const u32 xl = (x0 == 0u) ? a.nw - 1u : x0 - 1u;
+    const u64 ev = msc_ld<COH>(crow + (L.right ? xr : xl));
+    L.e = L.right ? (u32)ev : (u32)(ev >> 32);
     return L;
 }
 
@@ -443,6 +442,16 @@ __device__ __forceinline__ void msc_enqueue(MscQueueLds<R>& L, uint16_t* q, u32*
     }
 }
 
+// End of the calling wave, once its memory operations have completed (wave-uniform call
+// sites only). S_BARRIER waits only for the waves of the workgroup that have not ended (CDNA
+// ISA), so the remaining waves go on using __syncthreads; a barrier they pass afterwards also
+// orders the ended wave's stores (complete before it ended) before what follows it.
+__device__ __forceinline__ void msc_wave_end()
+{
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt, expcnt and lgkmcnt all zero
+    __builtin_amdgcn_endpgm();
+}
+
 // Queue compaction over the W = 256 * R words starting at `base`, one 256-thread workgroup
 // (all threads must call it; it contains block barriers).
 // Phase A: R rounds of one word per lane, CAP lockstep Philox calls unrolled without exit
@@ -453,6 +462,11 @@ __device__ __forceinline__ void msc_enqueue(MscQueueLds<R>& L, uint16_t* q, u32*
 //   round, instead of on every lane that finished before the slowest of its 64 words.
 // Tail: once at most 64 survivors remain, wave 0 finishes them, lockstep, or two level pairs
 // per call when at most 32 remain.
+// EXIT: a wave ends (msc_wave_end) as soon as no survivor work is left for it -- waves
+//   1..3 once the queue is shorter than their first slot, and before the tail -- instead of
+//   idling at the round barriers while other waves finish; its slot then serves another
+//   workgroup. Only for kernels that do nothing after the tile but in wave 0 (the caller's
+//   code after the call runs in the waves that are left: wave 0 always is).
 // ROWW: nw % 64 == 0 (every wave's 64 words lie in one row, see msc_classify_row).
 // Returns, when COUNT, this thread's share of the number of +1 spins written.
 // Phase A's loads of one tile, issued before any of its arithmetic (the words of every round
@@ -512,7 +526,7 @@ __device__ __forceinline__ void msc_phase_a_calls(const MscArgs& a, u32 base, in
 // FULL: every word of the tile exists (base + 256 R <= n_words), so no word needs the
 // past-the-end guards.
 template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, class PF, bool PRE = false,
-          bool FULL = false>
+          bool FULL = false, bool EXIT = false>
 __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes, u32 base,
                                                      MscQueueLds<R>& L,
                                                      const MscTileLoads<R, LAY>& T, PF prefetch,
@@ -582,6 +596,7 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
     u32 cur = 0;
     u32 lp = CAP;
     while (n > 64u && lp < 16u) {  // block-uniform
+        if (EXIT && wave != 0u && wave * 64u >= n) msc_wave_end();  // the queue only shrinks
         const u32 code = (u32)(codes >> (4u * lp)) & 15u;
         for (u32 i0 = wave * 64u; i0 < n; i0 += 256u) {  // wave-uniform
             // lanes past the queue's end (i >= n >= 1) repeat its last entry with nothing
@@ -612,6 +627,7 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
         cur ^= 1u;
         ++lp;
     }
+    if (EXIT && wave != 0u && (n <= 32u || wave * 64u >= n)) msc_wave_end();
     if (n <= 32u) {
         // Tail of at most 32 survivors, run by wave 0. Lane j < 32 owns survivor j and computes
         // its level pair lp; lane j + 32 computes level pair lp + 1 of the same word (any lane
@@ -675,13 +691,14 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
     return ones;
 }
 
-template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, bool COH = false>
+template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, bool COH = false,
+          bool EXIT = false>
 __device__ __forceinline__ u32 msc_queue_tile(const MscArgs& a, u64 codes, u32 base,
                                               MscQueueLds<R>& L, u32& calls, u32& slots,
                                               u32& words)
 {
     MscTileLoads<R, LAY> T;
     msc_tile_load<R, LAY, COH>(a, base, T);
-    return msc_queue_tile_loaded<R, CAP, LAY, STATS, COUNT, NT>(a, codes, base, L, T,
-                                                             MscNoPrefetch{}, calls, slots, words);
+    return msc_queue_tile_loaded<R, CAP, LAY, STATS, COUNT, NT, MscNoPrefetch, false, false, EXIT>(
+        a, codes, base, L, T, MscNoPrefetch{}, calls, slots, words);
 }
