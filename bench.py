@@ -305,7 +305,7 @@ def main() -> int:
     e0.record(sim.stream_ptr)
     sim.sweep(args.steps)
     e1.record(sim.stream_ptr)
-    while not e1.query():  # poll, so the end is seen without the runtime's blocking wake-up
+    while not e1.done():  # poll, so the end is seen without the runtime's blocking wake-up
         pass
     _lib.device_sync()
     wall = time.perf_counter() - t0

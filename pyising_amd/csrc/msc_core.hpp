@@ -479,17 +479,18 @@ struct MscTileLoads {
 };
 
 // Branch-free: words past the end load a clamped (valid) address instead, and their results
-// are discarded by the caller.
-template <int R, int LAY, bool COH = false>
+// are discarded by the caller. FULL: every word of the tile exists (no clamp).
+template <int R, int LAY, bool COH = false, bool FULL = false>
 __device__ __forceinline__ void msc_tile_load(const MscArgs& a, u32 base, MscTileLoads<R, LAY>& T)
 {
     const u32 tid = threadIdx.x;
+    const u32 wave_w = uni(tid & ~63u);  // the wave's first word in a round, scalar
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
         if (LAY != MSC_GENERAL) {
             // n_words is a multiple of 64: clamp the wave's first word
-            const u32 w0 = base + (u32)rr * 256u + (tid & ~63u);
-            const u32 wc = w0 < a.n_words ? w0 : a.n_words - 64u;
+            const u32 w0 = base + (u32)rr * 256u + wave_w;
+            const u32 wc = (FULL || w0 < a.n_words) ? w0 : a.n_words - 64u;
             if (LAY == MSC_ROWW) T.lr[rr] = msc_load_row<COH>(a, wc, tid & 63u);
             else T.ls[rr] = msc_load_seg<COH>(a, wc, tid & 63u);
         } else {

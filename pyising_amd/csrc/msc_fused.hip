@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void msc_fused_queue(FusedArgs f, u64 codes,
         if (!go) return;  // the whole workgroup: nothing written, counter not advanced
     }
     MscTileLoads<2, MSC_ROWW> T;
-    msc_tile_load<2, MSC_ROWW, true>(a, base, T);
+    msc_tile_load<2, MSC_ROWW, true, true>(a, base, T);
     msc_phase_a_calls<2, CAP>(a, base, 1, pre);  // covers the latency of the tile's loads
     u32 calls = 0, slots = 0, words = 0;
     msc_queue_tile_loaded<2, CAP, MSC_ROWW, STATS, false, true, MscNoPrefetch, true, true, !STATS>(

@@ -123,3 +123,31 @@ def test_bench_releases_explicitly(name):
     """The benchmark closes what it created before returning (no reliance on finalizers)."""
     text = (ROOT / name).read_text()
     assert ".close()" in text
+
+
+def test_bench_calls_only_existing_handle_methods():
+    """Every method bench.py calls on its HIP handles (events, streams, device buffers) exists
+    on the handle classes: the GPU path of bench.py runs on no CPU test, so a misspelt call
+    would first show on the GPU."""
+    import ast
+
+    from pyising_amd import _lib
+
+    tree = ast.parse((ROOT / "bench.py").read_text())
+    handles = {}  # variable name -> class, from `x = _lib.Event()` style assignments
+    for node in ast.walk(tree):
+        if isinstance(node, ast.Assign) and isinstance(node.value, (ast.Call, ast.Tuple)):
+            values = node.value.elts if isinstance(node.value, ast.Tuple) else [node.value]
+            targets = node.targets[0].elts if isinstance(node.targets[0], ast.Tuple) else [node.targets[0]]
+            for t, v in zip(targets, values):
+                if (isinstance(t, ast.Name) and isinstance(v, ast.Call)
+                        and isinstance(v.func, ast.Attribute) and isinstance(v.func.value, ast.Name)
+                        and v.func.value.id == "_lib" and v.func.attr in ("Event", "Stream", "DeviceBuffer")):
+                    handles[t.id] = getattr(_lib, v.func.attr)
+    assert {"e0", "e1"} <= set(handles), handles
+    called = [(n.func.value.id, n.func.attr) for n in ast.walk(tree)
+              if isinstance(n, ast.Call) and isinstance(n.func, ast.Attribute)
+              and isinstance(n.func.value, ast.Name) and n.func.value.id in handles]
+    assert called
+    missing = [(v, m) for v, m in called if not hasattr(handles[v], m)]
+    assert not missing, missing
