@@ -13,6 +13,7 @@ Poisson request stream, and the oracle timed on one host core as the CPU baselin
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import platform
@@ -301,6 +302,7 @@ def main() -> int:
     settle_sweeps, settle_ms = settle_clocks(sim, e0, e1, args.settle_ms)
     sim.sweep(args.warmup)
     _lib.device_sync()  # (the fused sweeps' status is checked with the observables below)
+    gc.disable()  # no collection pass inside the (sub-millisecond at --steps 20) timed region
     t0 = time.perf_counter()
     e0.record(sim.stream_ptr)
     sim.sweep(args.steps)
@@ -309,6 +311,7 @@ def main() -> int:
         pass
     _lib.device_sync()
     wall = time.perf_counter() - t0
+    gc.enable()
     gpu_ms = e0.elapsed_ms(e1)
     flips_per_ns = args.steps * n_spins / wall / 1e9
     launches = 2 * args.steps

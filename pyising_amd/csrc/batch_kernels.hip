@@ -140,18 +140,40 @@ __device__ __forceinline__ void batch_fold_moments(BatchAcc& A, const BatchDesc&
 }
 
 // One 512-word msc tile by queue compaction (4 unrolled calls, then packed rounds), with the
-// lattice's wave classification (block-uniform branch). COH: loads that miss in the L1.
-template <bool COH>
+// lattice's wave classification (block-uniform branch). COH: loads that miss in the L1. EX: how
+// waves without survivor work leave the tile (MscStay: they stay to its end).
+template <bool COH, class EX = MscStay>
 __device__ __forceinline__ u32 batch_msc_tile(const MscArgs& a, u64 codes, u32 base, u32 layout,
-                                              MscQueueLds<2>& lds)
+                                              MscQueueLds<2>& lds, EX ex = EX{})
 {
     u32 calls = 0, slots = 0, words = 0;
     if (layout == MSC_ROWW)
-        return msc_queue_tile<2, 4, MSC_ROWW, false, true, true, COH>(a, codes, base, lds, calls, slots, words);
+        return msc_queue_tile<2, 4, MSC_ROWW, false, true, true, COH, EX>(a, codes, base, lds, calls, slots, words, ex);
     if (layout == MSC_SEGW)
-        return msc_queue_tile<2, 4, MSC_SEGW, false, true, true, COH>(a, codes, base, lds, calls, slots, words);
-    return msc_queue_tile<2, 4, MSC_GENERAL, false, true, true, COH>(a, codes, base, lds, calls, slots, words);
+        return msc_queue_tile<2, 4, MSC_SEGW, false, true, true, COH, EX>(a, codes, base, lds, calls, slots, words, ex);
+    return msc_queue_tile<2, 4, MSC_GENERAL, false, true, true, COH, EX>(a, codes, base, lds, calls, slots, words, ex);
 }
+
+// Early leave of a one-tile workgroup's waves (MscLeave with a hand-over of the wave's +1
+// spins): the launch form stores the wave's count in its slot (nothing when moments are off),
+// the dependency-ordered form adds it to the tile's LDS total. The waves that stay hand over
+// their counts after the tile in the same way, so every wave's count is taken exactly once.
+struct BatchLeaveSlot {
+    static constexpr bool leave = true;
+    u32* dst;  // the tile's per-wave count slots, or null
+    __device__ void operator()(u32 ones) const
+    {
+        if (dst) wave_store_sum(dst, ones);
+    }
+};
+struct BatchLeavePart {
+    static constexpr bool leave = true;
+    u32* part;  // the tile's count in LDS, or null
+    __device__ void operator()(u32 ones) const
+    {
+        if (part && ones) atomicAdd(part, ones);
+    }
+};
 
 // One 256-word lockstep msc tile; returns the +1 spins written. With a whole-wave layout the
 // word count is a multiple of 64, so the validity test is wave-uniform.
@@ -209,9 +231,9 @@ enum : int { TILES_ALL = 0, TILES_LIGHT = 1, TILES_QUEUE = 2 };
 // spins written. Every thread of the workgroup calls it; callers that ran an earlier tile put a
 // barrier first (its threads may still read thr_lds). COH: lattice loads miss in the L1. SET:
 // the kinds compiled in (the caller only passes lattices of those kinds).
-template <bool COH, int SET = TILES_ALL>
+template <bool COH, int SET = TILES_ALL, class EX = MscStay>
 __device__ __forceinline__ u32 batch_tile(const BatchDesc& d, MscQueueLds<2>& lds, u32* thr_lds,
-                                          u32 local, u32 t, u32 color)
+                                          u32 local, u32 t, u32 color, EX ex = EX{})
 {
     if (SET != TILES_QUEUE && d.kind == BATCH_INT8) {
         if (d.ndim == 2) return d.hb ? batch_int8_v<2, 1, COH>(d, thr_lds, local, t, color)
@@ -230,9 +252,9 @@ __device__ __forceinline__ u32 batch_tile(const BatchDesc& d, MscQueueLds<2>& ld
     a.color = color;
     a.k0 = d.k0;
     a.k1 = d.k1;
-    if constexpr (SET == TILES_QUEUE) return batch_msc_tile<COH>(a, d.codes, local * 512u, d.layout, lds);
+    if constexpr (SET == TILES_QUEUE) return batch_msc_tile<COH>(a, d.codes, local * 512u, d.layout, lds, ex);
     else if constexpr (SET == TILES_LIGHT) return batch_msc_lockstep<COH>(a, d.codes, local, d.layout);
-    else return d.msc_queue ? batch_msc_tile<COH>(a, d.codes, local * 512u, d.layout, lds)
+    else return d.msc_queue ? batch_msc_tile<COH>(a, d.codes, local * 512u, d.layout, lds, ex)
                             : batch_msc_lockstep<COH>(a, d.codes, local, d.layout);
 }
 
@@ -274,10 +296,15 @@ __global__ __launch_bounds__(256) void batch_half(BatchKernelArgs ka, u32 color,
         if (sw >= d.limit) continue;  // past the lattice's sweep limit: the whole workgroup idles
         if (j) __syncthreads();  // an earlier tile of this workgroup may still read thr_lds
         const u32 local = tile - d.tile_begin;
-        const u32 plus = batch_tile<false>(d, lds, thr_lds, local, d.t0 + sw, color);
-        if (ka.moments)
-            wave_store_sum(ka.plus + ((size_t)par * 2 + color) * BATCH_WAVES * ka.total_tiles +
-                               (size_t)BATCH_WAVES * tile, plus);
+        u32* dst = ka.moments ? ka.plus + ((size_t)par * 2 + color) * BATCH_WAVES * ka.total_tiles +
+                                    (size_t)BATCH_WAVES * tile
+                              : nullptr;
+        u32 plus;
+        if constexpr (K == 1)  // nothing follows the tile: waves without work may leave
+            plus = batch_tile<false>(d, lds, thr_lds, local, d.t0 + sw, color, BatchLeaveSlot{dst});
+        else
+            plus = batch_tile<false>(d, lds, thr_lds, local, d.t0 + sw, color);
+        if (dst) wave_store_sum(dst, plus);
     }
 }
 
@@ -511,7 +538,12 @@ __global__ __launch_bounds__(256) void batch_fused(BatchFusedArgs f)
         const u64 e = K == 1 ? e0 : ents[i];
         const u32 tile = (u32)e, local = tile - d.tile_begin;
         if (i) __syncthreads();  // the previous tile's threads may still read thr_lds
-        const u32 plus = batch_tile<true, SET>(d, lds, thr_lds, local, d.t0 + sw, color);
+        u32 plus;
+        if constexpr (K == 1)  // only wave 0 works after the tile: waves without work may leave
+            plus = batch_tile<true, SET>(d, lds, thr_lds, local, d.t0 + sw, color,
+                                         BatchLeavePart{MOM ? &part[0] : nullptr});
+        else
+            plus = batch_tile<true, SET>(d, lds, thr_lds, local, d.t0 + sw, color);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are acknowledged
         if (MOM && plus) atomicAdd(&part[0], plus);  // LDS; part[0] is zero here
         __syncthreads();

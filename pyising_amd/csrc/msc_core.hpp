@@ -442,6 +442,17 @@ __device__ __forceinline__ void msc_enqueue(MscQueueLds<R>& L, uint16_t* q, u32*
     }
 }
 
+// Early end of the waves of a queue tile with no survivor work left: MscStay keeps every wave
+// to the end of the tile, MscLeave ends them (with nothing to hand over).
+struct MscStay {
+    static constexpr bool leave = false;
+    __device__ void operator()(u32) const {}
+};
+struct MscLeave {
+    static constexpr bool leave = true;
+    __device__ void operator()(u32) const {}
+};
+
 // End of the calling wave, once its memory operations have completed (wave-uniform call
 // sites only). S_BARRIER waits only for the waves of the workgroup that have not ended (CDNA
 // ISA), so the remaining waves go on using __syncthreads; a barrier they pass afterwards also
@@ -462,11 +473,12 @@ __device__ __forceinline__ void msc_wave_end()
 //   round, instead of on every lane that finished before the slowest of its 64 words.
 // Tail: once at most 64 survivors remain, wave 0 finishes them, lockstep, or two level pairs
 // per call when at most 32 remain.
-// EXIT: a wave ends (msc_wave_end) as soon as no survivor work is left for it -- waves
+// EX::leave: a wave ends (msc_wave_end) as soon as no survivor work is left for it -- waves
 //   1..3 once the queue is shorter than their first slot, and before the tail -- instead of
 //   idling at the round barriers while other waves finish; its slot then serves another
-//   workgroup. Only for kernels that do nothing after the tile but in wave 0 (the caller's
-//   code after the call runs in the waves that are left: wave 0 always is).
+//   workgroup. The hook ex(ones) runs first (all lanes of the wave), e.g. to hand over the
+//   wave's count of +1 spins. Only for kernels whose code after the tile needs no wave but
+//   wave 0 (it runs in the waves that are left; wave 0 always is).
 // ROWW: nw % 64 == 0 (every wave's 64 words lie in one row, see msc_classify_row).
 // Returns, when COUNT, this thread's share of the number of +1 spins written.
 // Phase A's loads of one tile, issued before any of its arithmetic (the words of every round
@@ -527,12 +539,13 @@ __device__ __forceinline__ void msc_phase_a_calls(const MscArgs& a, u32 base, in
 // FULL: every word of the tile exists (base + 256 R <= n_words), so no word needs the
 // past-the-end guards.
 template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, class PF, bool PRE = false,
-          bool FULL = false, bool EXIT = false>
+          bool FULL = false, class EX = MscStay>
 __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes, u32 base,
                                                      MscQueueLds<R>& L,
                                                      const MscTileLoads<R, LAY>& T, PF prefetch,
                                                      u32& calls, u32& slots, u32& words,
-                                                     const MscPhaseACalls<R, CAP>* pre = nullptr)
+                                                     const MscPhaseACalls<R, CAP>* pre = nullptr,
+                                                     EX ex = EX{})
 {
     static_assert(CAP >= 1 && CAP <= 15, "phase A levels");
     const u32 tid = threadIdx.x;
@@ -597,7 +610,10 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
     u32 cur = 0;
     u32 lp = CAP;
     while (n > 64u && lp < 16u) {  // block-uniform
-        if (EXIT && wave != 0u && wave * 64u >= n) msc_wave_end();  // the queue only shrinks
+        if (EX::leave && wave != 0u && wave * 64u >= n) {  // the queue only shrinks
+            ex(ones);
+            msc_wave_end();
+        }
         const u32 code = (u32)(codes >> (4u * lp)) & 15u;
         for (u32 i0 = wave * 64u; i0 < n; i0 += 256u) {  // wave-uniform
             // lanes past the queue's end (i >= n >= 1) repeat its last entry with nothing
@@ -628,7 +644,10 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
         cur ^= 1u;
         ++lp;
     }
-    if (EXIT && wave != 0u && (n <= 32u || wave * 64u >= n)) msc_wave_end();
+    if (EX::leave && wave != 0u && (n <= 32u || wave * 64u >= n)) {
+        ex(ones);
+        msc_wave_end();
+    }
     if (n <= 32u) {
         // Tail of at most 32 survivors, run by wave 0. Lane j < 32 owns survivor j and computes
         // its level pair lp; lane j + 32 computes level pair lp + 1 of the same word (any lane
@@ -693,13 +712,13 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
 }
 
 template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, bool COH = false,
-          bool EXIT = false>
+          class EX = MscStay>
 __device__ __forceinline__ u32 msc_queue_tile(const MscArgs& a, u64 codes, u32 base,
                                               MscQueueLds<R>& L, u32& calls, u32& slots,
-                                              u32& words)
+                                              u32& words, EX ex = EX{})
 {
     MscTileLoads<R, LAY> T;
     msc_tile_load<R, LAY, COH>(a, base, T);
-    return msc_queue_tile_loaded<R, CAP, LAY, STATS, COUNT, NT, MscNoPrefetch, false, false, EXIT>(
-        a, codes, base, L, T, MscNoPrefetch{}, calls, slots, words);
+    return msc_queue_tile_loaded<R, CAP, LAY, STATS, COUNT, NT, MscNoPrefetch, false, false, EX>(
+        a, codes, base, L, T, MscNoPrefetch{}, calls, slots, words, nullptr, ex);
 }

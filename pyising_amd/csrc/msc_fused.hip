@@ -41,6 +41,7 @@
 #include "placement.hpp"
 
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 // Sync workspace (u32 words): [0] failure bits (1: a wait gave up); [FUSED_HDR + i] half
@@ -142,7 +143,8 @@ __global__ __launch_bounds__(256) void msc_fused_queue(FusedArgs f, u64 codes,
     msc_tile_load<2, MSC_ROWW, true, true>(a, base, T);
     msc_phase_a_calls<2, CAP>(a, base, 1, pre);  // covers the latency of the tile's loads
     u32 calls = 0, slots = 0, words = 0;
-    msc_queue_tile_loaded<2, CAP, MSC_ROWW, STATS, false, true, MscNoPrefetch, true, true, !STATS>(
+    msc_queue_tile_loaded<2, CAP, MSC_ROWW, STATS, false, true, MscNoPrefetch, true, true,
+                          std::conditional_t<STATS, MscStay, MscLeave>>(
         a, codes, base, lds, T, MscNoPrefetch{}, calls, slots, words, &pre);
     if (STATS) msc_stats(stats, calls, slots, words);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are acknowledged

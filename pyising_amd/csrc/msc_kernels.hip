@@ -38,6 +38,7 @@
 
 #include <atomic>
 #include <mutex>
+#include <type_traits>
 
 // ---------------------------------------------------------------------------- kernels ----
 // stats (optional, diagnostics only): [0] += Philox calls the words needed, [1] += lane-slots
@@ -72,14 +73,15 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u64 codes, u32 
                                                      unsigned long long* stats)
 {
     __shared__ MscQueueLds<R> lds;
+    using Leave = std::conditional_t<STATS, MscStay, MscLeave>;  // the statistics need every wave
     u32 calls = 0, slots = 0, words = 0;
     if (R == 1 || blockIdx.x < n_big) {
         const u32 base = xcd_band_block(blockIdx.x, n_big) * (256u * R);
-        msc_queue_tile<R, CAP, LAY, STATS, false, true, false, !STATS>(a, codes, base, lds, calls,
-                                                                      slots, words);
+        msc_queue_tile<R, CAP, LAY, STATS, false, true, false, Leave>(a, codes, base, lds, calls,
+                                                                     slots, words);
     } else if constexpr (R == 2) {
         const u32 base = n_big * 512u + (blockIdx.x - n_big) * 256u;
-        msc_queue_tile<1, CAP, LAY, STATS, false, true, false, !STATS>(
+        msc_queue_tile<1, CAP, LAY, STATS, false, true, false, Leave>(
             a, codes, base, *reinterpret_cast<MscQueueLds<1>*>(&lds), calls, slots, words);
     }
     if (STATS) msc_stats(stats, calls, slots, words);
