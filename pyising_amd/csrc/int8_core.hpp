@@ -100,6 +100,18 @@ __device__ __forceinline__ void store_vec(int8_t* p, const Vec<V>& r)
 
 #define LOW_BITS 0x01010101u
 
+// 2 * bits + (u < thr ? 1 : 0) as one compare into VCC and one add with carry-in (the compiler
+// otherwise selects 0/1 into a register and shifts and ORs the bits together).
+__device__ __forceinline__ u32 shift_in_lt(u32 bits, u32 u, u32 thr)
+{
+    u32 r;
+    asm("v_cmp_lt_u32 vcc, %1, %2\n\tv_addc_co_u32 %0, vcc, %3, %3, vcc"
+        : "=v"(r)
+        : "v"(u), "v"(thr), "v"(bits)
+        : "vcc");
+    return r;
+}
+
 __device__ __forceinline__ u32 byte_sum(u32 v)  // sum of the 4 bytes (each <= 63)
 {
     return (v * LOW_BITS) >> 24;
@@ -189,10 +201,10 @@ __device__ __forceinline__ Vec<V> int8_decide(const Vec<V>& S, const Vec<V>& C, 
         u32 bits = 0;
 #pragma unroll
         for (int b = 3; b >= 0; --b) {
-            const u32 thr = *reinterpret_cast<const u32*>(thr_bytes + ((cnt >> (8 * b)) & 0xFFu));
-            bits = bits + bits + (uu[b] < thr ? 1u : 0u);
+            const u32 thr = *reinterpret_cast<const u32*>(thr_bytes + __builtin_amdgcn_ubfe(cnt, 8u * b, 8u));
+            bits = shift_in_lt(bits, uu[b], thr);
         }
-        u32 f01 = (bits * 0x204081u) & LOW_BITS;
+        u32 f01 = __umul24(bits, 0x204081u) & LOW_BITS;  // bits < 16
         if constexpr (!HB) f01 |= ((cnt + FORCE) >> 7) & LOW_BITS;
         const u32 m = (f01 << 8) - (f01 << 1);  // 0xFE in every flagged byte
         O.w[i] = HB ? ~m : (S.w[i] ^ m);
