@@ -159,6 +159,29 @@ __device__ __forceinline__ Vec<V> int8_horizontal(const Vec<V>& C, u32 e, u32 p)
     return H;
 }
 
+// The Philox outputs of the V / 4 quads of V consecutive colour sites whose first site has
+// colour index n0: quad q = n0 / 4 + i takes counter (q, t, colour, 0) (truncated to 32 bits).
+// They depend on no spin, so callers compute them while the spins are still being loaded.
+template <int V>
+struct Int8Draws {
+    PhiloxOut r[V / 4];
+};
+
+template <int V>
+__device__ __forceinline__ Int8Draws<V> int8_draws(u64 n0, u32 t, u32 color, u32 k0, u32 k1)
+{
+    Int8Draws<V> D;
+#pragma unroll
+    for (int i = 0; i < V / 4; ++i) {
+        const u32 q = (u32)((n0 >> 2) + (u64)i);
+        const u64 qm = (u64)q * PHILOX_M0;
+        D.r[i] = philox_lane((u32)(qm >> 32), (u32)qm, t, color, 0u, k0, k1);
+        // pinned where it is written, so that it covers the latency of the loads before it
+        asm volatile("" : "+v"(D.r[i].x), "+v"(D.r[i].y), "+v"(D.r[i].z), "+v"(D.r[i].w));
+    }
+    return D;
+}
+
 // The new spins of V consecutive colour sites of colour `color` whose first site has colour
 // index n0 (own spins S; the other colour at the same sites C, beside them H, in the rows
 // either side U and D and, in 3D, in the planes either side ZU and ZD).
@@ -170,44 +193,74 @@ __device__ __forceinline__ Vec<V> int8_horizontal(const Vec<V>& C, u32 e, u32 p)
 // ((b * 0x204081) & 0x01010101 moves bit b to byte b). Metropolis adds the forced flips
 // (count >= d) by a bytewise compare and flips with s ^ 0xFE; heat bath writes 0x01 or 0xFF.
 // minus (when COUNT): the number of -1 spins written.
+// One 4-site word: own spins s, the other colour's neighbours (c, h, u, d and in 3D zu, zd),
+// the quad's Philox output r; returns the new word.
+template <int NDIM, int HB>
+__device__ __forceinline__ u32 int8_decide_quad(u32 s, u32 c, u32 h, u32 u, u32 d, u32 zu, u32 zd,
+                                                const PhiloxOut& r, const uint8_t* thr_bytes)
+{
+    constexpr u32 K4 = 0x04040404u;
+    constexpr u32 FORCE = (128u - 4u * NDIM) * LOW_BITS;  // byte >= 4d sets its bit 7
+    const u32 uu[4] = {r.x, r.y, r.z, r.w};
+    u32 cnt;
+    if constexpr (HB) {
+        cnt = (c & K4) + (h & K4) + (u & K4) + (d & K4);
+        if constexpr (NDIM == 3) cnt += (zu & K4) + (zd & K4);
+    } else {
+        cnt = ((s ^ c) & K4) + ((s ^ h) & K4) + ((s ^ u) & K4) + ((s ^ d) & K4);
+        if constexpr (NDIM == 3) cnt += ((s ^ zu) & K4) + ((s ^ zd) & K4);
+    }
+    u32 bits = 0;
+#pragma unroll
+    for (int b = 3; b >= 0; --b) {
+        const u32 thr = *reinterpret_cast<const u32*>(thr_bytes + __builtin_amdgcn_ubfe(cnt, 8u * b, 8u));
+        bits = shift_in_lt(bits, uu[b], thr);
+    }
+    u32 f01 = __umul24(bits, 0x204081u) & LOW_BITS;  // bits < 16
+    if constexpr (!HB) f01 |= ((cnt + FORCE) >> 7) & LOW_BITS;
+    const u32 m = (f01 << 8) - (f01 << 1);  // 0xFE in every flagged byte
+    return HB ? ~m : (s ^ m);
+}
+
+// R: the quads' Philox outputs (int8_draws).
+template <int NDIM, int HB, int V, bool COUNT>
+__device__ __forceinline__ Vec<V> int8_decide_drawn(const Vec<V>& S, const Vec<V>& C,
+                                                    const Vec<V>& H, const Vec<V>& U,
+                                                    const Vec<V>& D, const Vec<V>& ZU,
+                                                    const Vec<V>& ZD, const Int8Draws<V>& R,
+                                                    const u32* thr_lds, u32& minus)
+{
+    const uint8_t* thr_bytes = reinterpret_cast<const uint8_t*>(thr_lds);
+    Vec<V> O;
+    minus = 0;
+#pragma unroll
+    for (int i = 0; i < V / 4; ++i) {
+        O.w[i] = int8_decide_quad<NDIM, HB>(S.w[i], C.w[i], H.w[i], U.w[i], D.w[i],
+                                            NDIM == 3 ? ZU.w[i] : 0u, NDIM == 3 ? ZD.w[i] : 0u,
+                                            R.r[i], thr_bytes);
+        if constexpr (COUNT) minus += __popc(O.w[i] & 0x80808080u);
+    }
+    return O;
+}
+
+// The same with each quad's Philox call made beside its decision.
 template <int NDIM, int HB, int V, bool COUNT>
 __device__ __forceinline__ Vec<V> int8_decide(const Vec<V>& S, const Vec<V>& C, const Vec<V>& H,
                                               const Vec<V>& U, const Vec<V>& D, const Vec<V>& ZU,
                                               const Vec<V>& ZD, u64 n0, const u32* thr_lds,
                                               u32 t, u32 color, u32 k0, u32 k1, u32& minus)
 {
-    constexpr int NWd = V / 4;
-    constexpr u32 K4 = 0x04040404u;
-    constexpr u32 FORCE = (128u - 4u * NDIM) * LOW_BITS;  // byte >= 4d sets its bit 7
     const uint8_t* thr_bytes = reinterpret_cast<const uint8_t*>(thr_lds);
     Vec<V> O;
     minus = 0;
 #pragma unroll
-    for (int i = 0; i < NWd; ++i) {
+    for (int i = 0; i < V / 4; ++i) {
         const u32 q = (u32)((n0 >> 2) + (u64)i);  // quad index, truncated to 32 bits
         const u64 qm = (u64)q * PHILOX_M0;
         const PhiloxOut r = philox_lane((u32)(qm >> 32), (u32)qm, t, color, 0u, k0, k1);
-        const u32 uu[4] = {r.x, r.y, r.z, r.w};
-        u32 cnt;
-        if constexpr (HB) {
-            cnt = (C.w[i] & K4) + (H.w[i] & K4) + (U.w[i] & K4) + (D.w[i] & K4);
-            if constexpr (NDIM == 3) cnt += (ZU.w[i] & K4) + (ZD.w[i] & K4);
-        } else {
-            const u32 s = S.w[i];
-            cnt = ((s ^ C.w[i]) & K4) + ((s ^ H.w[i]) & K4) + ((s ^ U.w[i]) & K4) +
-                  ((s ^ D.w[i]) & K4);
-            if constexpr (NDIM == 3) cnt += ((s ^ ZU.w[i]) & K4) + ((s ^ ZD.w[i]) & K4);
-        }
-        u32 bits = 0;
-#pragma unroll
-        for (int b = 3; b >= 0; --b) {
-            const u32 thr = *reinterpret_cast<const u32*>(thr_bytes + __builtin_amdgcn_ubfe(cnt, 8u * b, 8u));
-            bits = shift_in_lt(bits, uu[b], thr);
-        }
-        u32 f01 = __umul24(bits, 0x204081u) & LOW_BITS;  // bits < 16
-        if constexpr (!HB) f01 |= ((cnt + FORCE) >> 7) & LOW_BITS;
-        const u32 m = (f01 << 8) - (f01 << 1);  // 0xFE in every flagged byte
-        O.w[i] = HB ? ~m : (S.w[i] ^ m);
+        O.w[i] = int8_decide_quad<NDIM, HB>(S.w[i], C.w[i], H.w[i], U.w[i], D.w[i],
+                                            NDIM == 3 ? ZU.w[i] : 0u, NDIM == 3 ? ZD.w[i] : 0u,
+                                            r, thr_bytes);
         if constexpr (COUNT) minus += __popc(O.w[i] & 0x80808080u);
     }
     return O;
@@ -217,7 +270,9 @@ __device__ __forceinline__ Vec<V> int8_decide(const Vec<V>& S, const Vec<V>& C, 
 // global memory (int8_decide on the loaded neighbours). Returns the number of -1 spins written
 // (for magnetisation sums when COUNT).
 // COH: the other colour and the own sites are read with loads that miss in the L1.
-template <int NDIM, int HB, int V, bool COUNT, bool COH = false>
+// EARLY: the quads' Philox outputs are computed while the loads are in flight (their 4 V
+// registers then live across the loads: for kernels with registers to spare).
+template <int NDIM, int HB, int V, bool COUNT, bool COH = false, bool EARLY = false>
 __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
                                                  const int8_t* __restrict__ other, const Int8Geom& g,
                                                  const u32* thr_lds, u32 t, u32 color, u32 k0,
@@ -247,10 +302,19 @@ __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
     // horizontal neighbour: j + 1 when p == 1, j - 1 when p == 0 (one edge byte from memory)
     const u32 je = p ? ((j0 + V == g.lh) ? 0u : j0 + V) : ((j0 == 0) ? g.lh - 1 : j0 - 1);
     const u32 e = load_byte_at<COH>(other, rbase + je);
-    const Vec<V> H = int8_horizontal<V>(C, e, p);
     u32 minus = 0;
-    const Vec<V> O = int8_decide<NDIM, HB, V, COUNT>(S, C, H, U, D, ZU, ZD, n0, thr_lds, t, color,
-                                                     k0, k1, minus);
+    Vec<V> O;
+    if constexpr (EARLY) {
+        // the draws need no spin: computed while the loads above are in flight
+        const Int8Draws<V> R = int8_draws<V>(n0, t, color, k0, k1);
+        __builtin_amdgcn_sched_barrier(0);
+        const Vec<V> H = int8_horizontal<V>(C, e, p);
+        O = int8_decide_drawn<NDIM, HB, V, COUNT>(S, C, H, U, D, ZU, ZD, R, thr_lds, minus);
+    } else {
+        const Vec<V> H = int8_horizontal<V>(C, e, p);
+        O = int8_decide<NDIM, HB, V, COUNT>(S, C, H, U, D, ZU, ZD, n0, thr_lds, t, color, k0, k1,
+                                            minus);
+    }
     store_vec<V>(own + n0, O);
     return minus;
 }

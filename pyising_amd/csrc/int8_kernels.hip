@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void int8_half_kernel(int8_t* __restrict__ own
     const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
     const u32 ci = blk * blockDim.x + threadIdx.x;
     if (ci >= g.n_chunks) return;
-    int8_update_chunk<NDIM, HB, V, false>(own, other, g, thr_lds, t, color, k0, k1, ci);
+    int8_update_chunk<NDIM, HB, V, false, false, true>(own, other, g, thr_lds, t, color, k0, k1, ci);
 }
 
 
