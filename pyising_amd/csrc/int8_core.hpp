@@ -167,19 +167,26 @@ struct Int8Draws {
     PhiloxOut r[V / 4];
 };
 
+// q0: the first quad's index, n0 / 4 truncated to 32 bits (the counter wraps with it).
 template <int V>
-__device__ __forceinline__ Int8Draws<V> int8_draws(u64 n0, u32 t, u32 color, u32 k0, u32 k1)
+__device__ __forceinline__ Int8Draws<V> int8_draws_q(u32 q0, u32 t, u32 color, u32 k0, u32 k1)
 {
     Int8Draws<V> D;
 #pragma unroll
     for (int i = 0; i < V / 4; ++i) {
-        const u32 q = (u32)((n0 >> 2) + (u64)i);
+        const u32 q = q0 + (u32)i;
         const u64 qm = (u64)q * PHILOX_M0;
         D.r[i] = philox_lane((u32)(qm >> 32), (u32)qm, t, color, 0u, k0, k1);
         // pinned where it is written, so that it covers the latency of the loads before it
         asm volatile("" : "+v"(D.r[i].x), "+v"(D.r[i].y), "+v"(D.r[i].z), "+v"(D.r[i].w));
     }
     return D;
+}
+
+template <int V>
+__device__ __forceinline__ Int8Draws<V> int8_draws(u64 n0, u32 t, u32 color, u32 k0, u32 k1)
+{
+    return int8_draws_q<V>((u32)(n0 >> 2), t, color, k0, k1);
 }
 
 // The new spins of V consecutive colour sites of colour `color` whose first site has colour
@@ -316,5 +323,127 @@ __device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
                                             minus);
     }
     store_vec<V>(own + n0, O);
+    return minus;
+}
+
+// ------------------------------------------------------------- whole-row waves ----
+// Layouts in which a wave's 64 chunks (w0 .. w0 + 63, w0 a multiple of 64) share their row
+// arithmetic: I8_ROWW, chunks_row % 64 == 0 (the wave lies in one row: row, plane, parity
+// and the neighbour rows are scalar); I8_SEGW, chunks_row divides 64 and 64 / chunks_row rows
+// divide ly (the wave is whole rows of one plane: the plane and its neighbours are scalar,
+// lane l holds chunk l % chunks_row of row y0 + l / chunks_row). Loads take a scalar base
+// plus a 32-bit lane offset (a plane is below 4 GiB), and the horizontal edge byte comes from
+// the lane holding the neighbouring chunk (DPP shift in a row wave, ds_bpermute in a segment
+// wave) instead of a load. I8_GENERAL: one chunk's arithmetic per lane (int8_update_chunk).
+enum : int { I8_GENERAL = 0, I8_ROWW = 1, I8_SEGW = 2 };
+
+static inline int int8_layout(const Int8Geom& g)
+{
+    const u32 cr = g.chunks_row;
+    if (g.n_chunks % 64u || (unsigned long long)g.ly * g.lh >= (1ull << 32)) return I8_GENERAL;
+    if (cr % 64u == 0) return I8_ROWW;
+    if (64u % cr == 0 && g.ly % (64u / cr) == 0) return I8_SEGW;
+    return I8_GENERAL;
+}
+
+template <int NDIM, int HB, int V, bool COUNT, bool COH, bool EARLY, int LAY>
+__device__ __forceinline__ u32 int8_update_wave(int8_t* __restrict__ own,
+                                                const int8_t* __restrict__ other,
+                                                const Int8Geom& g, const u32* thr_lds, u32 t,
+                                                u32 color, u32 k0, u32 k1, u32 w0)
+{
+    static_assert(LAY == I8_ROWW || LAY == I8_SEGW, "whole-row wave layouts only");
+    constexpr int NWd = V / 4;
+    const u32 lane = threadIdx.x & 63u;
+    w0 = uni(w0);
+    u64 pbase;       // first byte of the wave's plane (SEGW) or row (ROWW): scalar
+    u32 off;         // this lane's chunk, bytes from pbase
+    u32 uoff, doff;  // the rows above and below, bytes from pbase (SEGW: within the plane)
+    u32 p;           // 1: colour sites at odd columns, horizontal neighbour j + 1
+    u32 e;           // the edge byte beyond this lane's vector
+    u32 z;
+    u32 c0 = 0;           // ROWW: the wave's first chunk in its row
+    const int8_t* ubase;  // ROWW: base pointers of the rows above and below
+    const int8_t* dbase;
+    if constexpr (LAY == I8_ROWW) {
+        const u32 row = fast_div(w0, g.div_chunks);
+        c0 = w0 - row * g.chunks_row;
+        z = fast_div(row, g.div_ly);
+        const u32 y = row - z * g.ly;
+        p = (y + z + color) & 1u;
+        pbase = (u64)row * g.lh;
+        const u32 yu = (y + 1u == g.ly) ? 0u : y + 1u;
+        const u32 yd = (y == 0u) ? g.ly - 1u : y - 1u;
+        ubase = other + ((u64)z * g.ly + yu) * g.lh;
+        dbase = other + ((u64)z * g.ly + yd) * g.lh;
+        off = (c0 + lane) * (u32)V;
+        uoff = off;
+        doff = off;
+    } else {
+        const u32 lg = (u32)__builtin_ctz(g.chunks_row);
+        const u32 row0 = w0 >> lg;
+        z = fast_div(row0, g.div_ly);
+        const u32 y0 = row0 - z * g.ly;
+        pbase = (u64)z * g.ly * g.lh;
+        const u32 c = lane & (g.chunks_row - 1u);
+        const u32 y = y0 + (lane >> lg);
+        p = (y + z + color) & 1u;
+        const u32 yoff = y * g.lh;
+        off = yoff + c * (u32)V;
+        uoff = (y + 1u == g.ly) ? c * (u32)V : off + g.lh;
+        doff = (y == 0u) ? (g.ly - 1u) * g.lh + c * (u32)V : off - g.lh;
+        ubase = other + pbase;
+        dbase = other + pbase;
+    }
+    int8_t* own_p = own + pbase;
+    const int8_t* oth_p = other + pbase;
+    const Vec<V> S = load_vec_at<V, COH>(own_p, off);
+    const Vec<V> C = load_vec_at<V, COH>(oth_p, off);
+    const Vec<V> U = load_vec_at<V, COH>(ubase, uoff);
+    const Vec<V> D = load_vec_at<V, COH>(dbase, doff);
+    Vec<V> ZU, ZD;  // 3D only
+    if constexpr (NDIM == 3) {
+        // the same row of the planes either side: pbase is a plane (SEGW) or row (ROWW) start,
+        // and the planes' offsets are whole multiples of a plane
+        const u64 plane = (u64)g.ly * g.lh;
+        const u64 zu = (z + 1u == g.lz) ? 0u : (u64)(z + 1u);
+        const u64 zd = (z == 0u) ? (u64)(g.lz - 1u) : (u64)(z - 1u);
+        const u64 in_plane = pbase - (u64)z * plane;  // 0 for SEGW
+        ZU = load_vec_at<V, COH>(other + zu * plane + in_plane, off);
+        ZD = load_vec_at<V, COH>(other + zd * plane + in_plane, off);
+    }
+    u32 ew = 0;  // ROWW: the word holding the edge byte of the wave's outer lane
+    if constexpr (LAY == I8_ROWW) {
+        const u32 eo = p ? (((c0 + 64u) * (u32)V == g.lh) ? 0u : (c0 + 64u) * (u32)V)
+                         : (c0 == 0u ? g.lh - 4u : c0 * (u32)V - 4u);
+        ew = load_vec_at<4, COH>(oth_p, eo).w[0];
+    }
+    const Int8Draws<V> R = EARLY ? int8_draws_q<V>((u32)(pbase >> 2) + (off >> 2), t, color, k0, k1)
+                                 : Int8Draws<V>{};
+    if constexpr (EARLY) __builtin_amdgcn_sched_barrier(0);
+    if constexpr (LAY == I8_ROWW) {
+        // lane i takes lane i + 1's first word (p = 1) or lane i - 1's last word (p = 0); the
+        // wave's outer lane keeps the loaded edge word
+        const u32 nb = p ? (u32)__builtin_amdgcn_update_dpp((int)ew, (int)C.w[0], 0x130, 0xF, 0xF, false)
+                         : (u32)__builtin_amdgcn_update_dpp((int)ew, (int)C.w[NWd - 1], 0x138, 0xF, 0xF, false);
+        e = p ? (nb & 0xFFu) : (nb >> 24);
+    } else {
+        // the lane holding the next (p = 1) or previous (p = 0) chunk of the same row, wrapping
+        // within the row (the torus); that lane has the same parity
+        const u32 cr = g.chunks_row;
+        const u32 c = lane & (cr - 1u);
+        const u32 src = (lane - c) + ((c + (p ? 1u : cr - 1u)) & (cr - 1u));
+        const u32 nb = (u32)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)(p ? C.w[0] : C.w[NWd - 1]));
+        e = p ? (nb & 0xFFu) : (nb >> 24);
+    }
+    const Vec<V> H = int8_horizontal<V>(C, e, p);
+    u32 minus = 0;
+    Vec<V> O;
+    if constexpr (EARLY)
+        O = int8_decide_drawn<NDIM, HB, V, COUNT>(S, C, H, U, D, ZU, ZD, R, thr_lds, minus);
+    else
+        O = int8_decide<NDIM, HB, V, COUNT>(S, C, H, U, D, ZU, ZD, pbase + off, thr_lds, t, color,
+                                            k0, k1, minus);
+    store_vec<V>(own_p + off, O);
     return minus;
 }

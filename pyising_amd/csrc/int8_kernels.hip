@@ -25,8 +25,9 @@
 // directly by the SWAR count byte (see int8_update_chunk).
 #include "int8_core.hpp"
 
-// One thread: sites j0 .. j0 + V - 1 of one row.
-template <int NDIM, int HB, int V>
+// One thread: sites j0 .. j0 + V - 1 of one row. LAY: I8_GENERAL (each lane's own row
+// arithmetic), I8_ROWW or I8_SEGW (a wave's row arithmetic shared, int8_update_wave).
+template <int NDIM, int HB, int V, int LAY>
 __global__ __launch_bounds__(256) void int8_half_kernel(int8_t* __restrict__ own,
                                                         const int8_t* __restrict__ other,
                                                         Int8Geom g, Int8Table tb, u32 t, u32 color,
@@ -36,9 +37,17 @@ __global__ __launch_bounds__(256) void int8_half_kernel(int8_t* __restrict__ own
     int8_fill_table<NDIM, HB>(thr_lds, tb);
     __syncthreads();
     const u32 blk = xcd_band_block(blockIdx.x, gridDim.x);
-    const u32 ci = blk * blockDim.x + threadIdx.x;
-    if (ci >= g.n_chunks) return;
-    int8_update_chunk<NDIM, HB, V, false, false, true>(own, other, g, thr_lds, t, color, k0, k1, ci);
+    if constexpr (LAY == I8_GENERAL) {
+        const u32 ci = blk * blockDim.x + threadIdx.x;
+        if (ci >= g.n_chunks) return;
+        int8_update_chunk<NDIM, HB, V, false, false, true>(own, other, g, thr_lds, t, color, k0,
+                                                           k1, ci);
+    } else {
+        const u32 w0 = blk * blockDim.x + uni(threadIdx.x & ~63u);  // n_chunks % 64 == 0
+        if (w0 >= g.n_chunks) return;                                // the whole wave
+        int8_update_wave<NDIM, HB, V, false, false, true, LAY>(own, other, g, thr_lds, t, color,
+                                                               k0, k1, w0);
+    }
 }
 
 
@@ -149,19 +158,33 @@ static int make_geom(long long rows, int lh, int ly, int lz, int ndim, Int8Geom*
     return ISING_OK;
 }
 
+template <int NDIM, int HB, int LAY>
+static void launch_half_lay(int V, dim3 grid, hipStream_t st, int8_t* own, const int8_t* other,
+                            const Int8Geom& g, const Int8Table& tb, u32 t, u32 color, u32 k0,
+                            u32 k1)
+{
+    if (V == 16)
+        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 16, LAY>), grid, dim3(256), 0, st, own,
+                           other, g, tb, t, color, k0, k1);
+    else if (V == 8)
+        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 8, LAY>), grid, dim3(256), 0, st, own,
+                           other, g, tb, t, color, k0, k1);
+    else
+        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 4, LAY>), grid, dim3(256), 0, st, own,
+                           other, g, tb, t, color, k0, k1);
+}
+
 template <int NDIM, int HB>
 static void launch_half(int V, dim3 grid, hipStream_t st, int8_t* own, const int8_t* other,
                         const Int8Geom& g, const Int8Table& tb, u32 t, u32 color, u32 k0, u32 k1)
 {
-    if (V == 16)
-        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 16>), grid, dim3(256), 0, st, own, other, g,
-                           tb, t, color, k0, k1);
-    else if (V == 8)
-        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 8>), grid, dim3(256), 0, st, own, other, g,
-                           tb, t, color, k0, k1);
+    const int lay = int8_layout(g);
+    if (lay == I8_ROWW)
+        launch_half_lay<NDIM, HB, I8_ROWW>(V, grid, st, own, other, g, tb, t, color, k0, k1);
+    else if (lay == I8_SEGW)
+        launch_half_lay<NDIM, HB, I8_SEGW>(V, grid, st, own, other, g, tb, t, color, k0, k1);
     else
-        hipLaunchKernelGGL((int8_half_kernel<NDIM, HB, 4>), grid, dim3(256), 0, st, own, other, g,
-                           tb, t, color, k0, k1);
+        launch_half_lay<NDIM, HB, I8_GENERAL>(V, grid, st, own, other, g, tb, t, color, k0, k1);
 }
 
 static int int8_half_impl(int8_t* own, const int8_t* other, long long rows, int lh, int ly, int lz,
