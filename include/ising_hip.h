@@ -108,6 +108,11 @@ int ising_fill_u64(uint64_t* d_dst, long long n, uint64_t value, void* stream);
 int ising_int8_half(int8_t* d_own, const int8_t* d_other, long long rows, int lh, int ly, int lz,
                     int ndim, int heatbath, const uint32_t* table, uint32_t t, uint32_t color,
                     uint32_t k0, uint32_t k1, void* stream);
+/* The per-half-sweep kernel's choice for a lattice: *vec = sites per lane (16, 8 or 4) and
+ * *layout = 0 (each lane's own row arithmetic), 1 (row waves: a wave's 64 lanes in one row) or
+ * 2 (segment waves: a wave is whole rows of one plane). Host only. No reference counterpart
+ * (kernel selection). */
+int ising_int8_kernel(long long rows, int lh, int ly, int lz, int ndim, int* vec, int* layout);
 /* n_sweeps full sweeps (colour 0 then 1, t = t0, t0+1, ...): IsingSimulation.sweep
  * (base.py:85-90) with all launches issued from C. */
 int ising_int8_sweeps(int8_t* d_c0, int8_t* d_c1, long long rows, int lh, int ly, int lz, int ndim,

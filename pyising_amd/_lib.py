@@ -59,6 +59,7 @@ SIGNATURES: dict[str, list] = {
     "ising_int8_resident_bytes": [_LL, _I, _I, _P],
     "ising_int8_sweeps_resident": [_P, _P, _LL, _I, _I, _P, _U32, _LL, _U32, _U32, _I, _P, _P, _P],
     "ising_int8_resident_status": [_P, _P, _P],
+    "ising_int8_kernel": [_LL, _I, _I, _I, _I, _P, _P],
     "ising_msc_resident_bytes": [_LL, _I, _I, _P],
     "ising_msc_sweeps_resident": [_P, _P, _LL, _I, _U32, _LL, _U32, _U32, _U32, _U32, _I, _P, _P, _P],
     "ising_msc_resident_status": [_P, _P, _P],

@@ -240,6 +240,15 @@ class HipInt8(_HipBase):
     def _upload_table(self) -> None:
         self._table_arr = np.ascontiguousarray(self.table, dtype=np.uint32)
 
+    def kernel(self) -> dict:
+        """The per-half-sweep kernel's choice for this lattice: {"vec": sites per lane,
+        "layout": 0 | 1 | 2, "name"} (resident sweeps, where they run, do not use it)."""
+        vec, lay = C.c_int(), C.c_int()
+        _lib.check(self._L.ising_int8_kernel(self.rows, self.lh, self.ly, self.lz, self.ndim,
+                                             C.byref(vec), C.byref(lay)), "int8_kernel")
+        names = {0: "per-lane rows", 1: "row waves", 2: "segment waves"}
+        return {"vec": vec.value, "layout": lay.value, "name": names[lay.value]}
+
     def _split(self, buf, tmp: bool) -> None:
         self._after_batch()
         _lib.check(self._L.ising_split_int8(buf.ptr, self.c[0].ptr, self.c[1].ptr, self.rows,

@@ -221,6 +221,18 @@ int ising_int8_half(int8_t* own, const int8_t* other, long long rows, int lh, in
                           as_stream(stream));
 }
 
+int ising_int8_kernel(long long rows, int lh, int ly, int lz, int ndim, int* vec, int* layout)
+{
+    if (!vec || !layout) return ising_fail(ISING_EINVAL, "int8_kernel: null pointer");
+    Int8Geom g;
+    int V;
+    int rc = make_geom(rows, lh, ly, lz, ndim, &g, &V);
+    if (rc) return rc;
+    *vec = V;
+    *layout = int8_layout(g);
+    return ISING_OK;
+}
+
 int ising_int8_sweeps(int8_t* c0, int8_t* c1, long long rows, int lh, int ly, int lz, int ndim,
                       int heatbath, const uint32_t* table, uint32_t t0, long long n_sweeps,
                       uint32_t k0, uint32_t k1, void* stream)

@@ -128,3 +128,30 @@ def test_msc_process_default_needs_no_gpu():
     finally:
         assert lib.ising_msc_config(3, 42) == 0
     assert default(16384, 128) == (0, 42)
+
+
+def test_int8_kernel_choice_needs_no_gpu():
+    """Sites per lane and wave layout of the int8 per-half-sweep kernel, host only: segment
+    waves when a row's chunks divide 64 and those rows divide the plane, row waves when a row
+    holds a multiple of 64 chunks, per-lane rows otherwise."""
+    import ctypes as C
+
+    lib = _lib.load_library()
+
+    def choice(shape):
+        ndim = len(shape)
+        lz = shape[0] if ndim == 3 else 1
+        v, lay = C.c_int(), C.c_int()
+        assert lib.ising_int8_kernel(shape[-2] * lz, shape[-1] // 2, shape[-2], lz, ndim,
+                                     C.byref(v), C.byref(lay)) == 0
+        return v.value, lay.value
+
+    assert choice((8192, 8192)) == (16, 1)      # 256 chunks per row
+    assert choice((256, 256, 256)) == (16, 2)   # 8 chunks per row, 8 rows per wave
+    assert choice((64, 64)) == (4, 2)
+    assert choice((256, 4096)) == (4, 1)
+    assert choice((16, 16, 16)) == (4, 0)       # 32 rows per wave > 16 rows per plane
+    assert choice((10, 12, 16)) == (4, 0)       # 240 chunks: not whole waves
+    v = C.c_int()
+    assert lib.ising_int8_kernel(10, 5, 10, 1, 2, C.byref(v), C.byref(v)) == -1  # lh % 4
+

@@ -37,6 +37,34 @@ def test_int8_bitwise_identical(hip, shape, algorithm):
     assert e == pytest.approx(O.observables_exact(ref.get_lattice())[1] / lat.size)
 
 
+# int8 per-half-sweep kernel layouts (ising_int8_kernel): segment waves (a wave is whole rows of
+# one plane), row waves (a wave's lanes in one row), per-lane rows; with the sites per lane the
+# host picks for each shape, and both parities of the first row (odd row counts included)
+INT8_LAYOUT_SHAPES = [((64, 64), 2), ((130, 256), 2), ((96, 128), 2), ((256, 4096), 1),
+                      ((512, 2048), 1), ((32, 8, 64), 2), ((6, 96, 256), 2), ((16, 16, 16), 0),
+                      ((10, 12, 16), 0)]
+
+
+@pytest.mark.parametrize("shape,layout", INT8_LAYOUT_SHAPES)
+@pytest.mark.parametrize("algorithm", ["metropolis", "heatbath"])
+def test_int8_half_sweep_layouts(hip, shape, layout, algorithm):
+    """One launch per half sweep (not the resident sweeps) in every wave layout, against the C
+    oracle after 1 and 7 sweeps."""
+    print(f"int8 layout {layout} {shape} {algorithm}", flush=True)
+    lat = O.random_lattice(shape, 21)
+    beta = 0.43 if len(shape) == 2 else 0.222
+    sim = hip.HipInt8(shape, beta, seed=2**33 + 5, algorithm=algorithm, init=lat)
+    sim.mode = hip.INT8_HALVES
+    assert sim.kernel()["layout"] == layout
+    ref = CO.CInt8(shape, beta, seed=2**33 + 5, algorithm=algorithm, init=lat)
+    for n in (1, 6):
+        sim.sweep(n)
+        ref.sweep(n)
+        np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+        assert sim.observables_exact() == tuple(ref.observables_exact())
+    sim.close()
+
+
 MSC_MODES = [(0, 42), (0, 32), (0, 52), (0, 62), (0, 44), (0, 34), (1, 0), (2, 0)]
 
 
