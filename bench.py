@@ -98,12 +98,14 @@ def cpu_baseline_openmp(min_repeat_s: float = 0.5, threads: int = 16) -> dict:
                       f"({total:.1f} s timed) on {threads} threads of {cpu_model()}"}
 
 
-def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
+def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20, repeats: int = 3) -> dict:
     """Binder temperature scan: 96 int8 lattices (L = 64..2048) and 32 msc lattices
     (L = 1024, 4096), 16 temperatures in [2.20, 2.34] each, seeds base + index, hot starts.
     Aggregate flips/ns of the batched scan (all lattices per launch; tiles, launch form and graph
     length picked by the autotuner, its timing outside the measured region) versus the plain
-    per-lattice loop (``sim.sweep(n)`` lattice after lattice)."""
+    per-lattice loop (``sim.sweep(n)`` lattice after lattice). Each side runs its timed work
+    once untimed first (same kernels, workspaces made, clocks up), then reports the median of
+    ``repeats`` timed runs."""
     from pyising_amd import _lib
     from pyising_amd.batch import BatchScan, temperature_scan_lattices
 
@@ -112,37 +114,49 @@ def batch_vs_plain(batch_sweeps: int = 200, plain_sweeps: int = 20) -> dict:
     scan = BatchScan.tuned(sims)  # autotuner: msc tile per L, launch form, graph length
     per_replay = scan.graph_sweeps
     graph = per_replay > 0
-    e0, e1 = _lib.Event(), _lib.Event()
-    if graph:
-        scan.replay(1)
-    else:
+    if not graph:
         per_replay = 50
-        scan.sweep(per_replay)
-    scan.synchronize()
     reps = max(1, batch_sweeps // per_replay)
-    e0.record(scan.stream.ptr)
-    if graph:
-        scan.replay(reps)
-    else:
-        scan.sweep(reps * per_replay)
-    e1.record(scan.stream.ptr)
-    batch_rate = reps * per_replay * n_spins / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+    e0, e1 = _lib.Event(), _lib.Event()
+
+    def batch_run() -> None:
+        if graph:
+            scan.replay(reps)
+        else:
+            scan.sweep(reps * per_replay)
+
+    batch_run()
+    scan.synchronize()
+    rates = []
+    for _ in range(repeats):
+        e0.record(scan.stream.ptr)
+        batch_run()
+        e1.record(scan.stream.ptr)
+        rates.append(reps * per_replay * n_spins / (e0.elapsed_ms(e1) * 1e-3) / 1e9)
+    batch_rate = float(np.median(rates))
     mom = scan.moments()
     tuning = scan.tuning
     scan.close()
-    for s in sims:
-        s.sweep(1)
-    _lib.device_sync()
-    t0 = time.perf_counter()
-    for s in sims:
-        s.sweep(plain_sweeps)
-    _lib.device_sync()
-    plain_rate = plain_sweeps * n_spins / (time.perf_counter() - t0) / 1e9
+
+    def plain_run() -> None:
+        for s in sims:
+            s.sweep(plain_sweeps)
+        _lib.device_sync()
+
+    plain_run()
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        plain_run()
+        times.append(time.perf_counter() - t0)
+    plain_rate = plain_sweeps * n_spins / float(np.median(times)) / 1e9
     for h in list(sims) + [e0, e1]:
         h.close()
     return {"lattices": len(sims), "spins": int(n_spins), "graph": bool(graph),
             "batched_flips_per_ns": round(batch_rate, 1), "plain_flips_per_ns": round(plain_rate, 1),
             "speedup": round(batch_rate / plain_rate, 2),
+            "timing": f"median of {repeats} timed runs each ({reps * per_replay} batch sweeps, "
+                      f"{plain_sweeps} sweeps per lattice in the plain loop) after one untimed run",
             "tuning": {"msc_tiles": {"x".join(map(str, k)): v for k, v in tuning.msc_tiles.items()},
                        "int8_vecs": {"x".join(map(str, k)): v for k, v in tuning.int8_vecs.items()},
                        "persistent": tuning.persistent, "graph_sweeps": tuning.graph_sweeps,

@@ -107,22 +107,44 @@ def _rate(scratch, t: BatchTuning, sweeps: int) -> float:
     try:
         n_spins = int(scan.n_spins.sum())
         e0, e1 = _lib.Event(), _lib.Event()
-        if graph_sweeps and scan.capture(graph_sweeps):
-            reps = max(1, sweeps // graph_sweeps)
-            scan.replay(1)
-            scan.synchronize()
+        graph = bool(graph_sweeps) and scan.capture(graph_sweeps)
+        reps = max(1, sweeps // graph_sweeps) if graph else 1
+        done = reps * graph_sweeps if graph else sweeps
+
+        def run() -> None:
+            if graph:
+                scan.replay(reps)
+            else:
+                scan.sweep(sweeps)
+
+        run()  # warm-up: the same work as a timed run
+        scan.synchronize()
+        best = 0.0
+        for _ in range(2):  # the better of two timed runs
             e0.record(scan.stream.ptr)
-            scan.replay(reps)
+            run()
             e1.record(scan.stream.ptr)
-            done = reps * graph_sweeps
-        else:
-            scan.sweep(2)
+            best = max(best, done * n_spins / (e0.elapsed_ms(e1) * 1e-3) / 1e9)
+        scan.synchronize()
+        for ev in (e0, e1):
+            ev.close()
+        return best
+    finally:
+        scan.close()
+
+
+def _settle(scratch, min_s: float = 0.3) -> None:
+    """Sweep the batch in its default form for at least ``min_s`` seconds of wall time."""
+    import time
+
+    from .batch import BatchScan
+
+    scan = BatchScan(scratch)
+    try:
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < min_s:
+            scan.sweep(10)
             scan.synchronize()
-            e0.record(scan.stream.ptr)
-            scan.sweep(sweeps)
-            e1.record(scan.stream.ptr)
-            done = sweeps
-        return done * n_spins / (e0.elapsed_ms(e1) * 1e-3) / 1e9
     finally:
         scan.close()
 
@@ -167,6 +189,9 @@ def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
         if r > best.flips_per_ns:
             best, best.flips_per_ns = cand, r
 
+    # the GPU's clocks need a few hundred milliseconds of load after idling: trials timed
+    # before that would favour whichever ran last
+    _settle(scratch)
     best.flips_per_ns = measure(best)
     for sh in msc_shapes:
         tiles = dict(best.msc_tiles)
@@ -182,6 +207,11 @@ def tune_batch(sims, sweeps: int = 50, use_cache: bool = True) -> BatchTuning:
         for k in TILES_PER_WG_CHOICES:
             if (persistent, k) != start:
                 consider(copy(persistent=persistent, tiles_per_workgroup=k))
+    # the msc tile choice again, under the chosen form (a tile's cost depends on the form)
+    for sh in msc_shapes:
+        tiles = dict(best.msc_tiles)
+        tiles[sh] = "queue" if tiles[sh] == "lockstep" else "lockstep"
+        consider(copy(msc_tiles=tiles))
     for g in GRAPH_CHOICES:
         if g != best.graph_sweeps:
             consider(copy(graph_sweeps=g))
