@@ -264,52 +264,127 @@ __device__ __forceinline__ PhiloxOut msc_philox_lanes(const MscArgs& a, const Ms
 // one level is 0, 1, a1 (thr4 bit 1, thr8 bit 0) or ~a1 (the reverse): kinds 0-3. With x, y
 // the uniform bits of the two levels: m = und & (x == t_A) stays undecided after level A,
 // und' = m & (y == t_B); the lanes accepted are those decided at a level whose threshold bit
-// is 1 (their uniform bit is then 0). When both levels have t = 1 every decided lane is
-// accepted: pw ^= und ^ und'. One to five v_bitop3 ops per 32-bit half, against two to six
-// for two one-level updates.
+// is 1 (their uniform bit is then 0).
 template <u32 B4, u32 B8>
 constexpr u32 msc_tkind()
 {
     return B4 ? (B8 ? 1u : 2u) : (B8 ? 3u : 0u);
 }
 
-template <u32 K>
-__device__ __forceinline__ u32 msc_teq(u32 x, u32 a)  // lanes whose bit equals the threshold bit
+// The definition, one level after the other (also the reference the forms below are proved
+// against at compile time).
+constexpr u32 msc_teq(u32 k, u32 x, u32 a)  // lanes whose bit equals the threshold bit
 {
-    if constexpr (K == 0) return ~x;
-    else if constexpr (K == 1) return x;
-    else if constexpr (K == 2) return ~(x ^ a);
-    else return x ^ a;
+    return k == 0 ? ~x : k == 1 ? x : k == 2 ? ~(x ^ a) : (x ^ a);
 }
-
-template <u32 K>
-__device__ __forceinline__ u32 msc_tacc(u32 u, u32 x, u32 a)  // lanes of u accepted here
+constexpr u32 msc_tbit(u32 k, u32 a)  // the threshold bit
 {
-    if constexpr (K == 0) return 0u;
-    else if constexpr (K == 1) return u & ~x;
-    else if constexpr (K == 2) return u & a & ~x;
-    else return u & ~a & ~x;
+    return k == 0 ? 0u : k == 1 ? ~0u : k == 2 ? a : ~a;
 }
-
-template <u32 KA, u32 KB>
-__device__ __forceinline__ void msc_two_levels(u32 x, u32 y, u32 a1, u32& pw, u32& und)
+constexpr void msc_two_levels_def(u32 ka, u32 kb, u32 x, u32 y, u32 a, u32& pw, u32& und)
 {
-    const u32 m = und & msc_teq<KA>(x, a1);
-    const u32 u2 = m & msc_teq<KB>(y, a1);
-    if constexpr (KA == 1 && KB == 1) pw = xor3(pw, und, u2);
-    else pw ^= msc_tacc<KA>(und, x, a1) ^ msc_tacc<KB>(m, y, a1);
+    const u32 m = und & msc_teq(ka, x, a);
+    const u32 u2 = m & msc_teq(kb, y, a);
+    pw ^= ((und ^ m) & msc_tbit(ka, a)) ^ ((m ^ u2) & msc_tbit(kb, a));
     und = u2;
 }
+
+// The same pair as explicit v_bitop3 ops. Per 32-bit half, with u = und: und' = u & G(x, y, a)
+// and pw' = pw ^ (u & T(x, y, a)), where G = (x == t_A) & (y == t_B) keeps the lanes both
+// levels leave undecided and T = t_A & ~x | (x == t_A) & t_B & ~y accepts. G and T are
+// three-input functions, so each is one op, and so are the products with u and the xor into
+// pw; ops fold where G or T ignore an input: und' in one op when G ignores a, pw' in one op
+// when T depends on a single input, none when T = 0 (no level accepts), xor3(pw, u, und')
+// when T = ~G (every decided lane is accepted). Two to eight ops per level pair for both
+// halves, against two to sixteen in the compiler's form of the definition.
+struct MscB3Emu {  // v_bitop3_b32 on the host: bit i of the result is bit (a_i b_i c_i) of IMM
+    template <u32 IMM>
+    static constexpr u32 op(u32 a, u32 b, u32 c)
+    {
+        u32 r = 0;
+        for (u32 i = 0; i < 32; ++i) {
+            const u32 sel = (((a >> i) & 1u) << 2) | (((b >> i) & 1u) << 1) | ((c >> i) & 1u);
+            r |= ((IMM >> sel) & 1u) << i;
+        }
+        return r;
+    }
+};
+struct MscB3Dev {
+    template <u32 IMM>
+    static __device__ __forceinline__ u32 op(u32 a, u32 b, u32 c) { return bop3(a, b, c, IMM); }
+};
+
+template <u32 KA, u32 KB>
+struct MscPairForm {
+    // truth tables over (x, y, a) = (0xF0, 0xCC, 0xAA)
+    static constexpr u32 G = (msc_teq(KA, 0xF0u, 0xAAu) & msc_teq(KB, 0xCCu, 0xAAu)) & 0xFFu;
+    static constexpr u32 T = ((msc_tbit(KA, 0xAAu) & ~0xF0u) |
+                              (msc_teq(KA, 0xF0u, 0xAAu) & msc_tbit(KB, 0xAAu) & ~0xCCu)) & 0xFFu;
+    static constexpr bool dep_x(u32 f) { return (((f >> 4) ^ f) & 0x0Fu) != 0u; }
+    static constexpr bool dep_y(u32 f) { return (((f >> 2) ^ f) & 0x33u) != 0u; }
+    static constexpr bool dep_a(u32 f) { return (((f >> 1) ^ f) & 0x55u) != 0u; }
+    // und' = bop3(u, x, y) when G ignores a: G restricted to (x, y) at a = 0, as (0xCC, 0xAA)
+    static constexpr u32 G_UXY = (0xF0u & msc_teq(KA, 0xCCu, 0u) & msc_teq(KB, 0xAAu, 0u)) & 0xFFu;
+    static constexpr int T_VARS = (dep_x(T) ? 1 : 0) + (dep_y(T) ? 1 : 0) + (dep_a(T) ? 1 : 0);
+    // pw' = bop3(pw, u, v) when T depends on v alone: T(v) as the pattern 0xAA
+    static constexpr u32 T_V = dep_x(T) ? ((msc_tbit(KA, 0u) & ~0xAAu) | (msc_teq(KA, 0xAAu, 0u) & msc_tbit(KB, 0u) & ~0u))
+                             : dep_y(T) ? ((msc_tbit(KA, 0u) & ~0u) | (msc_teq(KA, 0u, 0u) & msc_tbit(KB, 0u) & ~0xAAu))
+                                        : ((msc_tbit(KA, 0xAAu) & ~0u) | (msc_teq(KA, 0u, 0xAAu) & msc_tbit(KB, 0xAAu) & ~0u));
+    static constexpr u32 PW_UV = (0xF0u ^ (0xCCu & T_V)) & 0xFFu;      // pw ^ (u & T(v))
+    static constexpr u32 PW_UT = (0xF0u ^ (0xCCu & 0xAAu)) & 0xFFu;    // pw ^ (u & t)
+    static constexpr u32 XOR3 = (0xF0u ^ 0xCCu ^ 0xAAu) & 0xFFu;
+    static constexpr u32 AND = (0xF0u & 0xCCu) & 0xFFu;                // first two operands
+
+    template <class B>
+    static constexpr void apply(u32 x, u32 y, u32 a, u32& pw, u32& und)
+    {
+        const u32 u = und;
+        u32 u2 = 0;
+        if constexpr (!dep_a(G)) u2 = B::template op<G_UXY>(u, x, y);
+        else u2 = B::template op<AND>(u, B::template op<G>(x, y, a), 0u);
+        if constexpr (T == 0u) {
+        } else if constexpr (T == (~G & 0xFFu)) {
+            pw = B::template op<XOR3>(pw, u, u2);
+        } else if constexpr (T_VARS == 1) {
+            const u32 v = dep_x(T) ? x : dep_y(T) ? y : a;
+            pw = B::template op<PW_UV>(pw, u, v);
+        } else {
+            pw = B::template op<PW_UT>(pw, u, B::template op<T>(x, y, a));
+        }
+        und = u2;
+    }
+
+    // every combination of (u, x, y, a, pw) bits, one per bit position
+    static constexpr bool proved()
+    {
+        const u32 u = 0xFFFF0000u, x = 0xFF00FF00u, y = 0xF0F0F0F0u, a = 0xCCCCCCCCu, p = 0xAAAAAAAAu;
+        u32 pw0 = p, u0 = u, pw1 = p, u1 = u;
+        msc_two_levels_def(KA, KB, x, y, a, pw0, u0);
+        apply<MscB3Emu>(x, y, a, pw1, u1);
+        return pw0 == pw1 && u0 == u1;
+    }
+};
+
+template <u32 CODE>
+constexpr bool msc_pair_proved()
+{
+    return MscPairForm<msc_tkind<CODE & 1u, (CODE >> 1) & 1u>(),
+                       msc_tkind<(CODE >> 2) & 1u, (CODE >> 3) & 1u>()>::proved();
+}
+static_assert(msc_pair_proved<0>() && msc_pair_proved<1>() && msc_pair_proved<2>() && msc_pair_proved<3>() &&
+                  msc_pair_proved<4>() && msc_pair_proved<5>() && msc_pair_proved<6>() && msc_pair_proved<7>() &&
+                  msc_pair_proved<8>() && msc_pair_proved<9>() && msc_pair_proved<10>() && msc_pair_proved<11>() &&
+                  msc_pair_proved<12>() && msc_pair_proved<13>() && msc_pair_proved<14>() && msc_pair_proved<15>(),
+              "level-pair forms equal the level-by-level definition on every input");
 
 // CODE: bit 0 / 1 = thr4 / thr8 bit of level 2 lp, bits 2 / 3 the same for level 2 lp + 1.
 // Level 2 lp uses r.x (low half) and r.y (high half), level 2 lp + 1 uses r.z and r.w.
 template <u32 CODE>
 __device__ __forceinline__ void msc_pair(const PhiloxOut& r, MscState& s)
 {
-    constexpr u32 KA = msc_tkind<CODE & 1u, (CODE >> 1) & 1u>();
-    constexpr u32 KB = msc_tkind<(CODE >> 2) & 1u, (CODE >> 3) & 1u>();
-    msc_two_levels<KA, KB>(r.x, r.z, s.a1l, s.pwl, s.ul);
-    msc_two_levels<KA, KB>(r.y, r.w, s.a1h, s.pwh, s.uh);
+    using F = MscPairForm<msc_tkind<CODE & 1u, (CODE >> 1) & 1u>(), msc_tkind<(CODE >> 2) & 1u, (CODE >> 3) & 1u>()>;
+    F::template apply<MscB3Dev>(r.x, r.z, s.a1l, s.pwl, s.ul);
+    F::template apply<MscB3Dev>(r.y, r.w, s.a1h, s.pwh, s.uh);
 }
 
 __device__ __forceinline__ void msc_pair_switch(u32 code, const PhiloxOut& r, MscState& s)

@@ -51,3 +51,26 @@ def test_a_swapped_truth_table_is_caught():
         k = sum(o != n for n in (n1, n2, n3, n4))
         bad += msc_bits(o, n1, n2, n3, n4, imm) != (int(k <= 1), int(k == 1), o ^ int(k >= 2))
     assert bad > 0
+
+
+def test_level_pair_forms_are_proved_at_compile_time(tmp_path):
+    """The lazy comparison's level-pair forms (MscPairForm) are checked by static_assert in
+    msc_core.hpp against the level-by-level definition on every input combination; this
+    compiles the header for gfx950 (where hipcc is installed) so that a wrong form fails here,
+    and checks that the assertion still covers all 16 case codes."""
+    import shutil
+    import subprocess
+
+    text = CORE.read_text()
+    assert all(f"msc_pair_proved<{c}>()" in text for c in range(16))
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(hipcc).exists():
+        import pytest
+        pytest.skip("hipcc not installed")
+    src = tmp_path / "pair_proof.hip"
+    src.write_text('#include "msc_core.hpp"\n')
+    inc = CORE.parent
+    r = subprocess.run([hipcc, "--offload-arch=gfx950", "-std=c++17", "--cuda-device-only",
+                        "-fsyntax-only", f"-I{inc}", f"-I{inc.parents[1] / 'include'}", str(src)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
