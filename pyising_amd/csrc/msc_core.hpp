@@ -409,6 +409,39 @@ __device__ __forceinline__ void msc_pair_switch(u32 code, const PhiloxOut& r, Ms
     }
 }
 
+// The same level pair applied to N words at once (one per round of a tile, all at the same
+// level): the case switch, a chain of scalar compares and branches, runs once for the N.
+template <u32 CODE, int N>
+__device__ __forceinline__ void msc_pair_n(const PhiloxOut* const (&r)[N], MscState (&s)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) msc_pair<CODE>(*r[i], s[i]);
+}
+
+template <int N>
+__device__ __forceinline__ void msc_pair_switch_n(u32 code, const PhiloxOut* const (&r)[N],
+                                                  MscState (&s)[N])
+{
+    switch (code) {
+    case 0: msc_pair_n<0>(r, s); break;
+    case 1: msc_pair_n<1>(r, s); break;
+    case 2: msc_pair_n<2>(r, s); break;
+    case 3: msc_pair_n<3>(r, s); break;
+    case 4: msc_pair_n<4>(r, s); break;
+    case 5: msc_pair_n<5>(r, s); break;
+    case 6: msc_pair_n<6>(r, s); break;
+    case 7: msc_pair_n<7>(r, s); break;
+    case 8: msc_pair_n<8>(r, s); break;
+    case 9: msc_pair_n<9>(r, s); break;
+    case 10: msc_pair_n<10>(r, s); break;
+    case 11: msc_pair_n<11>(r, s); break;
+    case 12: msc_pair_n<12>(r, s); break;
+    case 13: msc_pair_n<13>(r, s); break;
+    case 14: msc_pair_n<14>(r, s); break;
+    default: msc_pair_n<15>(r, s); break;
+    }
+}
+
 // Nibble lp of the result = case code of levels 2 lp and 2 lp + 1 (host side, once per call).
 static inline u64 msc_level_codes(u32 thr4, u32 thr8)
 {
@@ -542,6 +575,10 @@ __device__ __forceinline__ void msc_wave_end()
 // (all threads must call it; it contains block barriers).
 // Phase A: R rounds of one word per lane, CAP lockstep Philox calls unrolled without exit
 //   tests (at T_c nearly every word needs them); decided words are stored, survivors parked.
+//   JOINT (implied by PRE): the calls of all R rounds are computed first, then every round is
+//   classified and each level pair's case switch (a chain of scalar compares and branches)
+//   runs once for the R words of a lane instead of once per round; the calls of every round
+//   are then live together (more VGPRs).
 // Phase B: rounds of exactly one call per survivor. Survivors are packed into full waves, all
 //   at the same level (so the level's case code stays wave-uniform); those still undecided
 //   are queued for the next round. A wave slot is wasted only in the last partial wave of a
@@ -614,7 +651,7 @@ __device__ __forceinline__ void msc_phase_a_calls(const MscArgs& a, u32 base, in
 // FULL: every word of the tile exists (base + 256 R <= n_words), so no word needs the
 // past-the-end guards.
 template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, class PF, bool PRE = false,
-          bool FULL = false, class EX = MscStay>
+          bool FULL = false, class EX = MscStay, bool JOINT = false>
 __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes, u32 base,
                                                      MscQueueLds<R>& L,
                                                      const MscTileLoads<R, LAY>& T, PF prefetch,
@@ -632,29 +669,19 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
     u32 ones = 0;
     if (tid < 18u) L.qn[tid] = 0u;
     __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < R; ++rr) {
+    // end of a phase-A word: stored if decided, else parked and queued
+    auto finish = [&](int rr, const MscState& s, bool valid) {
         const u32 slot = (u32)rr * 256u + tid;
-        const u32 w = base + slot;
-        const bool valid = FULL || w < a.n_words;
-        // the CAP Philox calls depend on the counter only: computed before the spin loads are
-        // consumed, they cover the memory latency (at the start of every workgroup above all)
-        PhiloxOut r[CAP];
-        if constexpr (PRE) {
-#pragma unroll
-            for (int lp = 0; lp < CAP; ++lp) r[lp] = pre->r[rr][lp];
-        } else {
-            const MscCounter ctr = msc_counter(w);
-#pragma unroll
-            for (int lp = 0; lp < CAP; ++lp) {
-                r[lp] = msc_philox(a, ctr, (u32)lp);
-                // an empty asm that "modifies" the outputs pins the call here: the optimiser
-                // would otherwise sink it below the classification and its wait for the loads
-                asm volatile("" : "+v"(r[lp].x), "+v"(r[lp].y), "+v"(r[lp].z), "+v"(r[lp].w));
-            }
+        const bool park = (s.ul | s.uh) != 0u;  // invalid lanes have und = 0
+        const u64 km = __ballot(park);
+        if (valid && !park) {
+            msc_store<NT>(a.own + base + slot, s);
+            if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
         }
-        __builtin_amdgcn_sched_barrier(0);
-        MscState s;
+        msc_enqueue(L, L.q[0], &L.qn[0], km, park, slot, s, true);
+    };
+    auto classify = [&](int rr, MscState& s, bool& valid) {
+        valid = FULL || base + (u32)rr * 256u + tid < a.n_words;
         if constexpr (LAY == MSC_ROWW) s = msc_classify_row(T.lr[rr]);
         else if constexpr (LAY == MSC_SEGW) s = msc_classify_seg(T.ls[rr], a.nw, lane);
         else s = msc_classify(T.lg[rr]);
@@ -663,21 +690,67 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
             s.uh = 0u;
         }
         if (valid) ++words;
+    };
+    if constexpr (PRE || JOINT) {
+        // Every round is classified first, then each level pair's case switch (a chain of
+        // scalar compares and branches) runs once for the words of all R rounds instead of once
+        // per round. Without PRE the calls of all rounds are computed here first (pinned where
+        // written, so that they cover the latency of the tile's loads).
+        MscPhaseACalls<R, CAP> own_calls;
+        if constexpr (!PRE) {
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) msc_phase_a_calls<R, CAP>(a, base, rr, own_calls);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const MscPhaseACalls<R, CAP>* calls_in = PRE ? pre : &own_calls;
+        MscState s[R];
+        bool valid[R];
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) classify(rr, s[rr], valid[rr]);
 #pragma unroll
         for (int lp = 0; lp < CAP; ++lp) {
-            if (STATS) {
-                calls += (s.ul | s.uh) != 0u ? 1u : 0u;
-                ++slots;
+            const PhiloxOut* rp[R];
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                rp[rr] = &calls_in->r[rr][lp];
+                if (STATS) {
+                    calls += (s[rr].ul | s[rr].uh) != 0u ? 1u : 0u;
+                    ++slots;
+                }
             }
-            msc_pair_switch((u32)(codes >> (4u * (u32)lp)) & 15u, r[lp], s);
+            msc_pair_switch_n<R>((u32)(codes >> (4u * (u32)lp)) & 15u, rp, s);
         }
-        const bool park = (s.ul | s.uh) != 0u;  // invalid lanes have und = 0
-        const u64 km = __ballot(park);
-        if (valid && !park) {
-            msc_store<NT>(a.own + w, s);
-            if (COUNT) ones += __popc(s.pwl) + __popc(s.pwh);
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) finish(rr, s[rr], valid[rr]);
+    } else {
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            // the CAP Philox calls depend on the counter only: computed before the spin loads
+            // are consumed, they cover the memory latency (at the start of every workgroup
+            // above all)
+            const MscCounter ctr = msc_counter(base + (u32)rr * 256u + tid);
+            PhiloxOut r[CAP];
+#pragma unroll
+            for (int lp = 0; lp < CAP; ++lp) {
+                r[lp] = msc_philox(a, ctr, (u32)lp);
+                // an empty asm that "modifies" the outputs pins the call here: the optimiser
+                // would otherwise sink it below the classification and its wait for the loads
+                asm volatile("" : "+v"(r[lp].x), "+v"(r[lp].y), "+v"(r[lp].z), "+v"(r[lp].w));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            MscState s;
+            bool valid;
+            classify(rr, s, valid);
+#pragma unroll
+            for (int lp = 0; lp < CAP; ++lp) {
+                if (STATS) {
+                    calls += (s.ul | s.uh) != 0u ? 1u : 0u;
+                    ++slots;
+                }
+                msc_pair_switch((u32)(codes >> (4u * (u32)lp)) & 15u, r[lp], s);
+            }
+            finish(rr, s, valid);
         }
-        msc_enqueue(L, L.q[0], &L.qn[0], km, park, slot, s, true);
     }
     __syncthreads();
     prefetch();  // e.g. the next tile's loads, which then complete while phase B runs
@@ -787,13 +860,13 @@ __device__ __forceinline__ u32 msc_queue_tile_loaded(const MscArgs& a, u64 codes
 }
 
 template <int R, int CAP, int LAY, bool STATS, bool COUNT, bool NT, bool COH = false,
-          class EX = MscStay>
+          class EX = MscStay, bool JOINT = false>
 __device__ __forceinline__ u32 msc_queue_tile(const MscArgs& a, u64 codes, u32 base,
                                               MscQueueLds<R>& L, u32& calls, u32& slots,
                                               u32& words, EX ex = EX{})
 {
     MscTileLoads<R, LAY> T;
     msc_tile_load<R, LAY, COH>(a, base, T);
-    return msc_queue_tile_loaded<R, CAP, LAY, STATS, COUNT, NT, MscNoPrefetch, false, false, EX>(
+    return msc_queue_tile_loaded<R, CAP, LAY, STATS, COUNT, NT, MscNoPrefetch, false, false, EX, JOINT>(
         a, codes, base, L, T, MscNoPrefetch{}, calls, slots, words, nullptr, ex);
 }

@@ -62,7 +62,9 @@ __global__ __launch_bounds__(256) void msc_half_lockstep(MscArgs a, u64 codes,
 
 // Queue compaction (the default): one workgroup per W = 256 * R words, CAP unrolled lockstep
 // calls, then rounds of one call per packed survivor (msc_queue_tile); streaming stores.
-// R = 2 uses 14 KB of LDS and 36 VGPRs: 8 workgroups (32 waves) per CU.
+// R = 2 uses 14 KB of LDS and at most 64 VGPRs: 8 workgroups (32 waves) per CU. R = 2 tiles
+// run phase A's level-pair switches once for both rounds (JOINT), which the R = 4 tiles'
+// register budget does not allow.
 static_assert(sizeof(MscQueueLds<1>) <= sizeof(MscQueueLds<2>), "graded tiles share the LDS");
 
 // Graded tail: workgroups [0, n_big) take 256 R-word tiles (XCD band mapped); the ones after,
@@ -77,8 +79,8 @@ __global__ __launch_bounds__(256) void msc_half_queue(MscArgs a, u64 codes, u32 
     u32 calls = 0, slots = 0, words = 0;
     if (R == 1 || blockIdx.x < n_big) {
         const u32 base = xcd_band_block(blockIdx.x, n_big) * (256u * R);
-        msc_queue_tile<R, CAP, LAY, STATS, false, true, false, Leave>(a, codes, base, lds, calls,
-                                                                     slots, words);
+        msc_queue_tile<R, CAP, LAY, STATS, false, true, false, Leave, R == 2>(
+            a, codes, base, lds, calls, slots, words);
     } else if constexpr (R == 2) {
         const u32 base = n_big * 512u + (blockIdx.x - n_big) * 256u;
         msc_queue_tile<1, CAP, LAY, STATS, false, true, false, Leave>(
