@@ -205,8 +205,8 @@ __device__ __forceinline__ u32 batch_int8_tile(const BatchDesc& d, const u32* th
 {
     const u32 ci = local * 256u + threadIdx.x;
     if (ci >= d.g.n_chunks) return 0u;
-    int8_t* own = (int8_t*)(color ? d.c1 : d.c0);
-    const int8_t* other = (const int8_t*)(color ? d.c0 : d.c1);
+    gi8* own = (gi8*)(color ? d.c1 : d.c0);
+    const gi8* other = (const gi8*)(color ? d.c0 : d.c1);
     const u32 minus = int8_update_chunk<NDIM, HB, V, true, COH>(own, other, d.g, thr_lds, t, color,
                                                                  d.k0, d.k1, ci);
     return (u32)V - minus;
@@ -242,8 +242,8 @@ __device__ __forceinline__ u32 batch_tile(const BatchDesc& d, MscQueueLds<2>& ld
                     : batch_int8_v<3, 0, COH>(d, thr_lds, local, t, color);
     }
     MscArgs a;
-    a.own = (u64*)(color ? d.c1 : d.c0);
-    a.other = (const u64*)(color ? d.c0 : d.c1);
+    a.own = (gu64*)(color ? d.c1 : d.c0);
+    a.other = (const gu64*)(color ? d.c0 : d.c1);
     a.rows = d.rows;
     a.nw = d.nw;
     a.n_words = d.n_words;

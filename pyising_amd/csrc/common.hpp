@@ -95,6 +95,29 @@ __device__ __forceinline__ PhiloxOut philox4x32_10(u32 c0, u32 c1, u32 c2, u32 c
 // Wave-uniform value forced into a scalar register.
 __device__ __forceinline__ u32 uni(u32 x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Device (global) memory as an explicit address space. A pointer that a kernel reads from
+// memory (a batch descriptor) is otherwise generic, and every access through it becomes a flat
+// instruction: it counts against both the vector-memory and the LDS wait counters, so a wait
+// for LDS also waits for it, and it takes no scalar base address. Kernel arguments are known to
+// be global, so only descriptor pointers need the cast; accesses must go through the typed
+// pointer (a cast back to a generic pointer loses the address space again).
+typedef __attribute__((address_space(1))) u64 gu64;
+typedef __attribute__((address_space(1))) int8_t gi8;
+
+// T in the address space of the byte type B (int8_t: generic, gi8: global), keeping const.
+template <class B, class T>
+struct in_as {
+    using type = T;
+};
+template <class T>
+struct in_as<gi8, T> {
+    using type = __attribute__((address_space(1))) T;
+};
+template <class B, class T>
+struct in_as<const B, T> {
+    using type = const typename in_as<B, T>::type;
+};
+
 // Philox4x32-10 of the counter (c0, c1, c2, c3) where only c0 varies across the wave and c1,
 // c2, c3, k0, k1 are wave-uniform (REQUIRED: callers pass sweep, colour, level and key). The
 // caller passes (hi0, lo0) = c0 * M0, which is loop-invariant over levels. The uniform halves

@@ -32,18 +32,25 @@ struct Vec<16> {
     u32 w[4];
 };
 
-template <int V>
-__device__ __forceinline__ Vec<V> load_vec(const int8_t* p)
+// B: int8_t (generic pointer: LDS, or a kernel argument, which the compiler knows is global) or
+// gi8 (a global pointer read from memory). The accesses use clang vector types, which copy by
+// value in any address space (HIP's uint4 has a copy constructor, which binds generic
+// references only).
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+typedef u32 u32x2 __attribute__((ext_vector_type(2)));
+
+template <int V, class B = int8_t>
+__device__ __forceinline__ Vec<V> load_vec(const B* p)
 {
     Vec<V> r;
     if constexpr (V == 16) {
-        const uint4 q = *reinterpret_cast<const uint4*>(p);
+        const u32x4 q = *reinterpret_cast<const typename in_as<B, u32x4>::type*>(p);
         r.w[0] = q.x; r.w[1] = q.y; r.w[2] = q.z; r.w[3] = q.w;
     } else if constexpr (V == 8) {
-        const uint2 q = *reinterpret_cast<const uint2*>(p);
+        const u32x2 q = *reinterpret_cast<const typename in_as<B, u32x2>::type*>(p);
         r.w[0] = q.x; r.w[1] = q.y;
     } else {
-        r.w[0] = *reinterpret_cast<const u32*>(p);
+        r.w[0] = *reinterpret_cast<const typename in_as<B, u32>::type*>(p);
     }
     return r;
 }
@@ -58,13 +65,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t coh_rsrc(const void* base)
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000);
 }
 
-template <int V, bool COH>
-__device__ __forceinline__ Vec<V> load_vec_at(const int8_t* base, u64 off)
+template <int V, bool COH, class B>
+__device__ __forceinline__ Vec<V> load_vec_at(const B* base, u64 off)
 {
     if constexpr (!COH) {
         return load_vec<V>(base + off);
     } else {
-        const __amdgpu_buffer_rsrc_t r = coh_rsrc(base);
+        const __amdgpu_buffer_rsrc_t r = coh_rsrc((const void*)base);
         Vec<V> v;
         if constexpr (V == 16) {
             const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, (u32)off, 0, COH_SC1);
@@ -79,22 +86,22 @@ __device__ __forceinline__ Vec<V> load_vec_at(const int8_t* base, u64 off)
     }
 }
 
-template <bool COH>
-__device__ __forceinline__ u32 load_byte_at(const int8_t* base, u64 off)
+template <bool COH, class B>
+__device__ __forceinline__ u32 load_byte_at(const B* base, u64 off)
 {
     if constexpr (!COH) return (u32)(uint8_t)base[off];
-    else return (u32)__builtin_amdgcn_raw_buffer_load_b8(coh_rsrc(base), (u32)off, 0, COH_SC1);
+    else return (u32)__builtin_amdgcn_raw_buffer_load_b8(coh_rsrc((const void*)base), (u32)off, 0, COH_SC1);
 }
 
-template <int V>
-__device__ __forceinline__ void store_vec(int8_t* p, const Vec<V>& r)
+template <int V, class B = int8_t>
+__device__ __forceinline__ void store_vec(B* p, const Vec<V>& r)
 {
     if constexpr (V == 16) {
-        *reinterpret_cast<uint4*>(p) = make_uint4(r.w[0], r.w[1], r.w[2], r.w[3]);
+        *reinterpret_cast<typename in_as<B, u32x4>::type*>(p) = u32x4{r.w[0], r.w[1], r.w[2], r.w[3]};
     } else if constexpr (V == 8) {
-        *reinterpret_cast<uint2*>(p) = make_uint2(r.w[0], r.w[1]);
+        *reinterpret_cast<typename in_as<B, u32x2>::type*>(p) = u32x2{r.w[0], r.w[1]};
     } else {
-        *reinterpret_cast<u32*>(p) = r.w[0];
+        *reinterpret_cast<typename in_as<B, u32>::type*>(p) = r.w[0];
     }
 }
 
@@ -279,9 +286,10 @@ __device__ __forceinline__ Vec<V> int8_decide(const Vec<V>& S, const Vec<V>& C, 
 // COH: the other colour and the own sites are read with loads that miss in the L1.
 // EARLY: the quads' Philox outputs are computed while the loads are in flight (their 4 V
 // registers then live across the loads: for kernels with registers to spare).
-template <int NDIM, int HB, int V, bool COUNT, bool COH = false, bool EARLY = false>
-__device__ __forceinline__ u32 int8_update_chunk(int8_t* __restrict__ own,
-                                                 const int8_t* __restrict__ other, const Int8Geom& g,
+// B: the byte type of the lattice pointers (int8_t or gi8, see load_vec).
+template <int NDIM, int HB, int V, bool COUNT, bool COH = false, bool EARLY = false, class B = int8_t>
+__device__ __forceinline__ u32 int8_update_chunk(B* __restrict__ own,
+                                                 const B* __restrict__ other, const Int8Geom& g,
                                                  const u32* thr_lds, u32 t, u32 color, u32 k0,
                                                  u32 k1, u32 ci)
 {

@@ -32,12 +32,11 @@ __global__ __launch_bounds__(RS_THREADS) void int8_resident_kernel(Int8ResidentA
     int8_fill_table<2, HB>(thr_lds, a.tb);  // ordered before its first use by rs_run's barrier
     const u32 lh = a.bands.row_bytes, R = a.bands.R, T = a.bands.T, rows = a.bands.rows;
     const u32 per_row = lh / RS_V;
-    rs_run(a.bands, rs_lds, a.c0, a.c1, &go, [&](int8_t** lc, u32 hh, u32 ext, u32 first) {
+    rs_run(a.bands, rs_lds, a.c0, a.c1, &go,
+           [&](int8_t* own, const int8_t* other, u32 hh, u32 ext, u32 first) {
         const u32 color = hh & 1u;
         const u32 t = a.t0 + (hh >> 1);
         const u32 row_lo = T - ext, n_rows = R + 2u * ext;
-        int8_t* own = lc[color];
-        const int8_t* other = lc[color ^ 1u];
         for (u32 it = threadIdx.x; it < n_rows * per_row; it += RS_THREADS) {
             const u32 rr = fast_div(it, a.div_row);
             const u32 row = row_lo + rr;  // LDS row, 1 <= row <= R + 2T - 2

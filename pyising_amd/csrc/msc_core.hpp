@@ -6,8 +6,8 @@
 #include "common.hpp"
 
 struct MscArgs {
-    u64* own;
-    const u64* other;
+    gu64* own;  // global memory (kernel arguments and descriptor pointers are cast to it)
+    const gu64* other;
     u32 rows, nw;
     u32 n_words;  // rows * nw (< 2^32, checked on the host)
     u32 t, color, k0, k1;
@@ -85,7 +85,7 @@ struct MscRowLoad {
 // the CU's L1: for kernels in which another workgroup of the same launch may have rewritten
 // the word after this CU cached it (the fused sweeps); otherwise an ordinary cached load.
 template <bool COH>
-__device__ __forceinline__ u64 msc_ld(const u64* p)
+__device__ __forceinline__ u64 msc_ld(const gu64* p)
 {
     if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else return *p;
@@ -99,7 +99,7 @@ __device__ __forceinline__ MscRowLoad msc_load_row(const MscArgs& a, u32 w0, u32
     const u32 x0 = w0 - y * a.nw;
     const u32 yu = (y + 1u == a.rows) ? 0u : y + 1u;
     const u32 yd = (y == 0u) ? a.rows - 1u : y - 1u;
-    const u64* crow = a.other + (u64)y * a.nw;
+    const gu64* crow = a.other + (u64)y * a.nw;
     MscRowLoad L;
     L.o = msc_ld<COH>(a.own + w0 + lane);
     L.c = msc_ld<COH>(crow + x0 + lane);
@@ -494,7 +494,7 @@ __device__ __forceinline__ void msc_stats(unsigned long long* stats, u32 calls, 
 // Store of a finished word. NT: non-temporal (streaming) store, so the new spins leave the
 // L2 during the kernel instead of in the write-back at the kernel boundary.
 template <bool NT>
-__device__ __forceinline__ void msc_store(u64* p, const MscState& s)
+__device__ __forceinline__ void msc_store(gu64* p, const MscState& s)
 {
     const u64 v = (u64)s.pwl | ((u64)s.pwh << 32);
     if (NT) __builtin_nontemporal_store(v, p);

@@ -111,8 +111,8 @@ __global__ __launch_bounds__(256) void msc_fused_queue(FusedArgs f, u64 codes,
     const bool device_scope = !f.one_l2 && (k == 0u || k + 1u == f.per_band);
     MscArgs a;
     const u32 color = h & 1u;
-    a.own = color ? f.w1 : f.w0;
-    a.other = color ? f.w0 : f.w1;
+    a.own = (gu64*)(color ? f.w1 : f.w0);
+    a.other = (const gu64*)(color ? f.w0 : f.w1);
     a.rows = f.rows;
     a.nw = f.nw;
     a.n_words = f.n_words;

@@ -246,8 +246,8 @@ static int msc_half_impl(u64* own, const u64* other, long long rows, int nw, u32
     if ((rc = msc_mode_ok(mode, k))) return rc;
     msc_resolve(&mode, &k, (u32)(rows * nw));
     MscArgs a;
-    a.own = own;
-    a.other = other;
+    a.own = (gu64*)own;
+    a.other = (const gu64*)other;
     a.rows = (u32)rows;
     a.nw = (u32)nw;
     a.n_words = (u32)(rows * nw);

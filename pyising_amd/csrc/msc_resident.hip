@@ -36,14 +36,14 @@ __global__ __launch_bounds__(RS_THREADS) void msc_resident_kernel(MscResidentArg
     const u32 nw = a.nw, R = a.bands.R, T = a.bands.T, rows = a.bands.rows;
     const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     rs_run(a.bands, rs_lds, (int8_t*)a.w0, (int8_t*)a.w1, &go,
-           [&](int8_t** lc, u32 hh, u32 ext, u32 first) {
+           [&](int8_t* own_lds, const int8_t* other_lds, u32 hh, u32 ext, u32 first) {
         MscArgs m;
         m.color = hh & 1u;
         m.t = a.t0 + (hh >> 1);
         m.k0 = a.k0;
         m.k1 = a.k1;
-        u64* own = reinterpret_cast<u64*>(lc[m.color]);
-        const u64* other = reinterpret_cast<const u64*>(lc[m.color ^ 1u]);
+        u64* own = reinterpret_cast<u64*>(own_lds);
+        const u64* other = reinterpret_cast<const u64*>(other_lds);
         const u32 row_lo = T - ext, n = (R + 2u * ext) * nw;
         // whole waves (the lazy loop ballots across the wave); lanes past the end idle
         for (u32 base = wave * 64u; base < n; base += RS_THREADS) {

@@ -101,10 +101,14 @@ __device__ bool rs_wait(const ResidentBands& a, u32 up, u32 dn, u32 target)
 }
 
 // The band of the calling workgroup and its neighbours, plus the round loop.
-// UPDATE(lc, hh, ext, first) runs half sweep hh of the launch (colour hh & 1, sweep hh >> 1 of
-// the call) on LDS rows T - ext .. T + R + ext - 1, i.e. the band and ext rows either side
-// (lc[2] = the LDS colour arrays, LDS row r = global row (first + r) mod rows), and ends with a
-// workgroup barrier. Returns after writing the band back, or early if a wait gave up.
+// UPDATE(own, other, hh, ext, first) runs half sweep hh of the launch (colour hh & 1, sweep
+// hh >> 1 of the call) on LDS rows T - ext .. T + R + ext - 1, i.e. the band and ext rows either
+// side (own, other = the LDS arrays of colour hh & 1 and of the other colour; LDS row r = global
+// row (first + r) mod rows), and ends with a workgroup barrier. Returns after writing the band
+// back, or early if a wait gave up.
+// The two pointers are offsets from the LDS base, not entries of an array of pointers: an
+// array indexed by the colour hides their address space from the compiler, which then reaches
+// the band through flat loads and stores (slower, and waiting on both memory counters).
 template <class UPDATE>
 __device__ __forceinline__ void rs_run(const ResidentBands& a, int8_t* lds, int8_t* g0,
                                        int8_t* g1, u32* go, UPDATE update)
@@ -149,7 +153,11 @@ __device__ __forceinline__ void rs_run(const ResidentBands& a, int8_t* lds, int8
             }
             __syncthreads();
         }
-        for (u32 i = 0; i < m; ++i) update(lc, h0 + i, m - 1u - i, first);
+        for (u32 i = 0; i < m; ++i) {
+            const u32 hh = h0 + i;
+            const size_t own_off = (size_t)(hh & 1u) * nl * rb;
+            update(lds + own_off, lds + ((size_t)nl * rb - own_off), hh, m - 1u - i, first);
+        }
         // publish the round: this band's first and last T rows of both colours (for the next
         // round's halos), or, after the last round, only the count
         const u32 par = r & 1u;
