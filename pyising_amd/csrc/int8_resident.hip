@@ -97,6 +97,7 @@ int ising_int8_sweeps_resident(int8_t* d_c0, int8_t* d_c1, long long rows, int l
     a.bands.T = pl.T;
     a.bands.B = pl.B;
     a.bands.one_l2 = pl.place == 0 ? 1u : 0u;
+    a.bands.tagged = pl.tagged;
     a.bands.ws = ws;
     a.c0 = d_c0;
     a.c1 = d_c1;

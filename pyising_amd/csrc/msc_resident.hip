@@ -111,6 +111,7 @@ int ising_msc_sweeps_resident(uint64_t* d_w0, uint64_t* d_w1, long long rows, in
     a.bands.T = pl.T;
     a.bands.B = pl.B;
     a.bands.one_l2 = pl.place == 0 ? 1u : 0u;
+    a.bands.tagged = pl.tagged;
     a.bands.ws = ws;
     a.w0 = d_w0;
     a.w1 = d_w1;

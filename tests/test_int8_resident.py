@@ -35,6 +35,7 @@ def _pair(hip, shape, beta, alg, seed, bands=0, mode=None):
     ((1024, 1024), "metropolis", 0.4406868),
     ((512, 1024), "heatbath", 1 / 2.269),
     ((2048, 2048), "metropolis", 1 / 2.269),
+    ((256, 4096), "heatbath", 1 / 2.269),  # 2048-byte rows: edge rows exchanged as rows
 ])
 def test_resident_matches_oracle(hip, shape, alg, beta):
     """Calls of 1, 2, 3, 8 and 25 sweeps (rounds cut short, one round, many rounds), each
