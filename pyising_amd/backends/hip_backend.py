@@ -34,6 +34,8 @@ MSC_RESIDENT = 5
 INT8_DEFAULT, INT8_HALVES, INT8_RESIDENT = -1, 0, 1
 #: ISING_OBS_SCRATCH_BYTES of ising_hip.h: partial sums of the observable reductions
 OBS_SCRATCH_BYTES = 16384
+#: the observables result: (M, E) as int64 and a status word read back with them
+OBS_OUT_BYTES = 24
 
 
 def _full_lattice(lat: np.ndarray, shape) -> np.ndarray:
@@ -95,15 +97,15 @@ class _HipBase(IsingSimulation):
         self._L = _lib.lib()
         self.k0, self.k1 = split_seed(self.seed)
         self._obs_scratch = self._alloc(OBS_SCRATCH_BYTES)
-        self._obs_out = self._alloc(16)
+        self._obs_out = self._alloc(OBS_OUT_BYTES)
 
     @staticmethod
     def state_bytes(shape, variant: str) -> list[int]:
         """Sizes of the device allocations a simulation of ``shape`` makes through its
-        allocator: the reduction scratch, the 16-byte (M, E) result and two colour arrays."""
+        allocator: the reduction scratch, the (M, E) result and two colour arrays."""
         n = int(np.prod(shape))
         colour = n // 16 if variant == "msc" else n // 2
-        return [OBS_SCRATCH_BYTES, 16, colour, colour]
+        return [OBS_SCRATCH_BYTES, OBS_OUT_BYTES, colour, colour]
 
     #: the batch (BatchScan or ReservedBatch) this lattice belongs to, while it does
     _batch = None
@@ -158,6 +160,16 @@ class _HipBase(IsingSimulation):
         out = np.zeros(2, dtype=np.int64)
         self._obs_out.download(out, self.stream_ptr)
         return int(out[0]), int(out[1])
+
+    def _read_obs_with_status(self, status_ptr) -> tuple[int, int, int]:
+        """(M, E), already computed into the result buffer, and the 32-bit status word at
+        ``status_ptr`` (a device copy into the buffer's third slot), in one read-back: one host
+        wait where reading the status on its own would add a second."""
+        _lib.check(self._L.ising_memcpy_d2d(self._obs_out.ptr + 16, status_ptr, 4,
+                                            self.stream_ptr), "memcpy_d2d")
+        out = np.zeros(3, dtype=np.int64)
+        self._obs_out.download(out, self.stream_ptr)
+        return int(out[0]), int(out[1]), int(out[2]) & 0xFFFFFFFF
 
 
 class HipInt8(_HipBase):
@@ -219,11 +231,15 @@ class HipInt8(_HipBase):
                                                       self.stream_ptr), "resident_status")
         self._rs_unchecked = False
         if failed.value:
-            _lib.check(self._L.ising_fill_u64(self._rs.ptr, self._rs.nbytes // 8, 0,
-                                              self.stream_ptr), "fill")
-            self._rs_epoch.value = 0
-            raise _lib.HipError(f"resident int8 sweep failed (status {failed.value}): the "
-                                "lattice is invalid; sweep with mode=INT8_HALVES instead")
+            self._resident_failed(failed.value)
+
+    def _resident_failed(self, status: int) -> None:
+        """Zero the resident workspace again and raise (a resident launch reported `status`)."""
+        _lib.check(self._L.ising_fill_u64(self._rs.ptr, self._rs.nbytes // 8, 0,
+                                          self.stream_ptr), "fill")
+        self._rs_epoch.value = 0
+        raise _lib.HipError(f"resident int8 sweep failed (status {status}): the "
+                            "lattice is invalid; sweep with mode=INT8_HALVES instead")
 
     def synchronize(self) -> None:
         super().synchronize()
@@ -321,8 +337,14 @@ class HipInt8(_HipBase):
             self._obs_scratch.ptr, out_ptr, self.stream_ptr), "int8_obs")
 
     def observables_exact(self) -> tuple[int, int]:
-        self.check_resident()
         self._obs_into(self._obs_out.ptr)
+        if self._rs_unchecked and self._rs is not None:
+            # the resident launches' status comes back with (M, E)
+            m, e, status = self._read_obs_with_status(self._rs.ptr)
+            self._rs_unchecked = False
+            if status:
+                self._resident_failed(status)
+            return m, e
         return self._read_obs()
 
 
@@ -395,11 +417,15 @@ class HipMultiSpin(_HipBase):
                                                      self.stream_ptr), "resident_status")
         self._rs_unchecked = False
         if failed.value:
-            _lib.check(self._L.ising_fill_u64(self._rs.ptr, self._rs.nbytes // 8, 0,
-                                              self.stream_ptr), "fill")
-            self._rs_epoch.value = 0
-            raise _lib.HipError(f"resident msc sweep failed (status {failed.value}): the "
-                                "lattice is invalid; sweep with mode=MSC_COMPACT instead")
+            self._resident_failed(failed.value)
+
+    def _resident_failed(self, status: int) -> None:
+        """Zero the resident workspace again and raise (a resident launch reported `status`)."""
+        _lib.check(self._L.ising_fill_u64(self._rs.ptr, self._rs.nbytes // 8, 0,
+                                          self.stream_ptr), "fill")
+        self._rs_epoch.value = 0
+        raise _lib.HipError(f"resident msc sweep failed (status {status}): the "
+                            "lattice is invalid; sweep with mode=MSC_COMPACT instead")
 
     def _sync_buffer(self):
         """The fused sweeps' workspace (per-tile completion counters), zeroed once; its counter
@@ -547,9 +573,16 @@ class HipMultiSpin(_HipBase):
 
     def observables_exact(self) -> tuple[int, int]:
         self._obs_into(self._obs_out.ptr)
+        if self._rs_unchecked and self._rs is not None:
+            # the resident launches' status comes back with (M, E)
+            m, e, status = self._read_obs_with_status(self._rs.ptr)
+            self._rs_unchecked = False
+            self.check_sync()
+            if status:
+                self._resident_failed(status)
+            return m, e
         out = self._read_obs()
         self.check_sync()
-        self.check_resident()
         return out
 
     def philox_stats(self, sweeps: int = 2) -> dict:

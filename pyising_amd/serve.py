@@ -70,10 +70,10 @@ class Request:
     def allocations(self) -> list[int]:
         """The device allocations of its simulation: the reduction scratch, the (M, E) result
         and the two colour arrays (int8: N/2 bytes each; msc: N/16 bytes each)."""
-        from .backends.hip_backend import OBS_SCRATCH_BYTES
+        from .backends.hip_backend import OBS_OUT_BYTES, OBS_SCRATCH_BYTES
 
         colour = self.n_spins // 16 if self.variant == "msc" else self.n_spins // 2
-        return [OBS_SCRATCH_BYTES, 16, colour, colour]
+        return [OBS_SCRATCH_BYTES, OBS_OUT_BYTES, colour, colour]
 
     def footprint(self) -> int:
         """Device bytes held while admitted (every allocation 256-byte aligned)."""

@@ -195,3 +195,31 @@ def test_msc_not_applicable(hip):
     big.sweep(2)
     assert big._rs is None
     big.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["int8", "msc"])
+def test_failure_bit_comes_back_with_observables(hip, variant):
+    """A failure bit in the resident workspace, set from the host as a bounded wait that gave up
+    sets it, is reported by the next observables_exact (read back with M and E), which zeroes
+    the workspace; sweeps from a reset lattice then continue bit for bit. Every kernel ends
+    normally: the waits see the bit and leave."""
+    from pyising_amd import _lib
+
+    if variant == "int8":
+        sim, ref = _pair(hip, (1024, 1024), 0.44, "metropolis", seed=12)
+    else:
+        sim, ref = _msc_pair(hip, (1024, 1024), 0.44, seed=12)
+    sim.sweep(2)
+    sim.synchronize()
+    _lib.check(sim._L.ising_fill_u64(sim._rs.ptr, 1, 1, sim.stream_ptr), "fill")
+    sim.sweep(3)
+    with pytest.raises(_lib.HipError, match=f"resident {variant} sweep failed"):
+        sim.observables_exact()
+    sim.set_lattice(ref.get_lattice())
+    ref.t = sim.t
+    sim.sweep(4)
+    ref.sweep(4)
+    np.testing.assert_array_equal(sim.get_lattice(), ref.get_lattice())
+    assert sim.observables_exact() == tuple(ref.observables_exact())
+    sim.close()

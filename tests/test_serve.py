@@ -75,7 +75,7 @@ def test_arena_random_churn_keeps_invariants():
 def test_footprint_is_the_aligned_state():
     from pyising_amd.backends.hip_backend import OBS_SCRATCH_BYTES as OBS
 
-    r = _req(0, L=64)  # int8 2D: 2 colour arrays of 64*32 bytes, reduction scratch, 16 B result
+    r = _req(0, L=64)  # int8 2D: 2 colour arrays of 64*32 bytes, reduction scratch, 24 B result
     assert r.footprint() == 2 * 2048 + OBS + ALIGN
     m = _req(1, L=256, variant="msc")  # 2 x 256*2 words of 8 B
     assert m.footprint() == 2 * 4096 + OBS + ALIGN
