@@ -133,7 +133,9 @@ int ising_int8_sweeps_resident(int8_t* d_c0, int8_t* d_c1, long long rows, int l
                                uint32_t k1, int bands, uint32_t* d_ws, uint32_t* epoch,
                                void* stream);
 /* Failure bits of the resident launches since the workspace was zeroed (synchronizes the
- * stream): 1 = a bounded wait for a neighbour band gave up; the lattice is then invalid. */
+ * stream): 1 = a bounded wait for a neighbour band gave up; the lattice is then invalid. The
+ * bits are the workspace's first 32-bit word, so a caller may instead copy that word on the
+ * stream together with other results and read both with one host wait. */
 int ising_int8_resident_status(const uint32_t* d_ws, int* failed, void* stream);
 /* Exact (M, E) into out2[0..1] (device int64); d_scratch2 = ISING_OBS_SCRATCH_BYTES of device
  * memory (any contents).
@@ -207,7 +209,8 @@ int ising_msc_sweeps_resident(uint64_t* d_w0, uint64_t* d_w1, long long rows, in
                               uint32_t thr8, int bands, uint32_t* d_ws, uint32_t* epoch,
                               void* stream);
 /* Failure bits of the resident launches since the workspace was zeroed (synchronizes the
- * stream): 1 = a bounded wait for a neighbour band gave up; the lattice is then invalid. */
+ * stream): 1 = a bounded wait for a neighbour band gave up; the lattice is then invalid. As for
+ * int8, the bits are the workspace's first 32-bit word. */
 int ising_msc_resident_status(const uint32_t* d_ws, int* failed, void* stream);
 /* Bytes of the sync workspace of a rows x (128 nw) lattice (host only). No reference
  * counterpart. */
