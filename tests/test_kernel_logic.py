@@ -74,3 +74,127 @@ def test_level_pair_forms_are_proved_at_compile_time(tmp_path):
                         "-fsyntax-only", f"-I{inc}", f"-I{inc.parents[1] / 'include'}", str(src)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+# ------------------------------------------------ eight lanes per word (msc_resident.hip) ----
+# The resident msc sweeps with small bands give a word eight lanes, lane j evaluating level pair
+# j; each lane forms the pair's masks G (undecided through both levels) and T (accepted) for a
+# word undecided everywhere, and the segment combines them with DPP moves inside the 8-lane
+# segment (rs_lanes8_combine): quad_perm scans, quad totals, row_shr:4 on banks 1 and 3,
+# row_shr:1, row_half_mirror. The lane moves are emulated here as the gfx9 DPP controls define
+# them, and the combined update must equal the level pairs applied one after the other (the
+# definition: the first level whose bit differs from the threshold bit decides, accepting when
+# the threshold bit is 1).
+_M32 = 0xFFFFFFFF
+
+
+def _quad(v, sel):
+    return [v[(i & ~3) + sel[i & 3]] for i in range(64)]
+
+
+def _row_shr(v, d, banks, old):
+    out = list(old)
+    for i in range(64):
+        li = i % 16
+        if (banks >> (li // 4)) & 1 and li >= d:
+            out[i] = v[i - d]
+    return out
+
+
+def _half_mirror(v):
+    return [v[(i & ~7) + 7 - (i & 7)] for i in range(64)]
+
+
+def _combine(G, T):
+    q = [a & b for a, b in zip(G, _quad(G, [0, 0, 1, 2]))]
+    q = [a & b for a, b in zip(q, _quad(q, [0, 1, 0, 1]))]
+    qt = [a & b for a, b in zip(G, _quad(G, [1, 0, 3, 2]))]
+    qt = [a & b for a, b in zip(qt, _quad(qt, [2, 3, 0, 1]))]
+    Q = [a & b for a, b in zip(q, _row_shr(qt, 4, 0b1010, [_M32] * 64))]
+    E = _row_shr(Q, 1, 0b1111, [_M32] * 64)
+    E = [_M32 if i % 8 == 0 else e for i, e in enumerate(E)]
+    acc = [a & b for a, b in zip(E, T)]
+    a = [x | y for x, y in zip(acc, _quad(acc, [1, 0, 3, 2]))]
+    a = [x | y for x, y in zip(a, _quad(a, [2, 3, 0, 1]))]
+    A = [x | y for x, y in zip(a, _half_mirror(a))]
+    Qall = [x & y for x, y in zip(qt, _half_mirror(qt))]
+    return A, Qall
+
+
+def _threshold(bit4, bit8, a1):
+    """The threshold bit of every lane at one level: thr4's bit where a1, thr8's elsewhere."""
+    return ((a1 if bit4 else 0) | ((~a1 & _M32) if bit8 else 0)) & _M32
+
+
+def _pairs_in_order(und, pw, a1, xs, ys, codes):
+    for j in range(8):
+        c = codes[j]
+        for x, tb in ((xs[j], _threshold(c & 1, c >> 1 & 1, a1)),
+                      (ys[j], _threshold(c >> 2 & 1, c >> 3 & 1, a1))):
+            decided = und & (x ^ tb)
+            pw ^= decided & tb  # x = 0 < threshold bit 1: accepted
+            und &= ~decided & _M32
+    return und, pw
+
+
+def _lane_masks(code, x, y, a1):
+    """rs_pair_masks of msc_resident.hip."""
+    ta = _threshold(code & 1, code >> 1 & 1, a1)
+    tb = _threshold(code >> 2 & 1, code >> 3 & 1, a1)
+    e = x ^ ta
+    G = ~e & ~(y ^ tb) & _M32
+    T = (ta & ~x | (~e & tb & ~y)) & _M32
+    return G, T
+
+
+def _check_segments(seed, plant=None):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    bad = 0
+    for trial in range(60):
+        und = [int(v) for v in rng.integers(0, 2**32, 8)]
+        if trial % 3 == 0:
+            und = [_M32] * 8
+        pw = [int(v) for v in rng.integers(0, 2**32, 8)]
+        a1 = [int(v) for v in rng.integers(0, 2**32, 8)]
+        xs = rng.integers(0, 2**32, (8, 8)).tolist()
+        ys = rng.integers(0, 2**32, (8, 8)).tolist()
+        if trial % 2:  # few bits differ from the thresholds: words decided late
+            xs = [[x & int(m) for x, m in zip(r, rng.integers(0, 2**32, 8))] for r in xs]
+        codes = rng.integers(0, 16, (8, 8)).tolist()
+        G, T = [0] * 64, [0] * 64
+        for lane in range(64):
+            w, j = divmod(lane, 8)
+            G[lane], T[lane] = _lane_masks(codes[w][j], xs[w][j], ys[w][j], a1[w])
+        if plant == "shift":  # exclusive scan without the segment-start reset
+            A, Qall = _combine_planted(G, T)
+        else:
+            A, Qall = _combine(G, T)
+        for w in range(8):
+            want = _pairs_in_order(und[w], pw[w], a1[w], xs[w], ys[w], codes[w])
+            for lane in range(8 * w, 8 * w + 8):
+                got = (und[w] & Qall[lane], pw[w] ^ (und[w] & A[lane]))
+                bad += got != want
+    return bad
+
+
+def _combine_planted(G, T):
+    q = [a & b for a, b in zip(G, _quad(G, [0, 0, 1, 2]))]
+    q = [a & b for a, b in zip(q, _quad(q, [0, 1, 0, 1]))]
+    qt = [a & b for a, b in zip(G, _quad(G, [1, 0, 3, 2]))]
+    qt = [a & b for a, b in zip(qt, _quad(qt, [2, 3, 0, 1]))]
+    Q = [a & b for a, b in zip(q, _row_shr(qt, 4, 0b1010, [_M32] * 64))]
+    E = _row_shr(Q, 1, 0b1111, [_M32] * 64)  # lane 8 of a row keeps the other segment's value
+    acc = [a & b for a, b in zip(E, T)]
+    a = [x | y for x, y in zip(acc, _quad(acc, [1, 0, 3, 2]))]
+    a = [x | y for x, y in zip(a, _quad(a, [2, 3, 0, 1]))]
+    return [x | y for x, y in zip(a, _half_mirror(a))], [x & y for x, y in zip(qt, _half_mirror(qt))]
+
+
+def test_eight_lane_combine_equals_pairs_in_order():
+    assert _check_segments(5) == 0
+
+
+def test_eight_lane_combine_catches_a_segment_leak():
+    assert _check_segments(6, plant="shift") > 0
