@@ -23,6 +23,56 @@ struct Int8ResidentArgs {
     Int8Table tb;
 };
 
+// Whole-row waves (lh = 64 RS_V times a power of two, e.g. L = 1024, 2048): a wave holds 64
+// consecutive chunks of one row, so the row, its parity and its LDS offsets are wave-uniform
+// and the horizontal edge byte comes from the lane holding the neighbouring chunk (DPP). With
+// one wave per row (ONE) the row wraps inside the wave (a rotate: the torus); else the wave's
+// outer lane takes the edge word from one uniform LDS load. own, other: the LDS colour arrays;
+// rows row_lo .. row_lo + n_rows - 1 of the band.
+template <int HB, bool ONE>
+__device__ __forceinline__ void int8_rs_rows(const Int8ResidentArgs& a, int8_t* own,
+                                             const int8_t* other, const u32* thr_lds, u32 row_lo,
+                                             u32 n_rows, u32 first, u32 color, u32 t)
+{
+    const u32 lh = a.bands.row_bytes, rows = a.bands.rows;
+    const u32 per_row = lh / RS_V;
+    const u32 lg = ONE ? 0u : (u32)__builtin_ctz(per_row / 64u);
+    const u32 wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    for (u32 rw = wave; rw < (n_rows << lg); rw += RS_THREADS / 64u) {
+        const u32 rr = ONE ? rw : uni(rw >> lg);
+        const u32 c0 = ONE ? 0u : uni((rw - (rr << lg)) * 64u);  // the wave's first chunk
+        const u32 row = row_lo + rr;
+        u32 y = first + row;
+        if (y >= rows) y -= rows;
+        const u32 p = uni((y + color) & 1u);
+        const u32 j0 = (c0 + lane) * RS_V;
+        const size_t at = (size_t)row * lh + j0;
+        const Vec<RS_V> S = load_vec<RS_V>(own + at);
+        const Vec<RS_V> C = load_vec<RS_V>(other + at);
+        const Vec<RS_V> U = load_vec<RS_V>(other + at + lh);  // row y + 1
+        const Vec<RS_V> D = load_vec<RS_V>(other + at - lh);  // row y - 1
+        // p = 1: first byte of the next chunk (lane i reads lane i + 1); p = 0: last byte of the
+        // previous chunk (lane i reads lane i - 1)
+        u32 nb;
+        if constexpr (ONE) {
+            nb = p ? (u32)__builtin_amdgcn_update_dpp(0, (int)C.w[0], 0x134, 0xF, 0xF, false)  // wave_rol:1
+                   : (u32)__builtin_amdgcn_update_dpp(0, (int)C.w[RS_V / 4 - 1], 0x13C, 0xF, 0xF, false);  // wave_ror:1
+        } else {
+            const u32 eo = p ? ((c0 + 64u == per_row) ? 0u : (c0 + 64u) * RS_V)
+                             : (c0 == 0u ? lh - 4u : c0 * RS_V - 4u);
+            const u32 ew = *reinterpret_cast<const u32*>(other + (size_t)row * lh + eo);
+            nb = p ? (u32)__builtin_amdgcn_update_dpp((int)ew, (int)C.w[0], 0x130, 0xF, 0xF, false)  // wave_shl:1
+                   : (u32)__builtin_amdgcn_update_dpp((int)ew, (int)C.w[RS_V / 4 - 1], 0x138, 0xF, 0xF, false);  // wave_shr:1
+        }
+        const u32 e = p ? (nb & 0xFFu) : (nb >> 24);
+        const Vec<RS_V> H = int8_horizontal<RS_V>(C, e, p);
+        u32 minus = 0;
+        const Vec<RS_V> O = int8_decide<2, HB, RS_V, false>(S, C, H, U, D, U, D, (u64)y * lh + j0,
+                                                            thr_lds, t, color, a.k0, a.k1, minus);
+        store_vec<RS_V>(own + at, O);
+    }
+}
+
 template <int HB>
 __global__ __launch_bounds__(RS_THREADS) void int8_resident_kernel(Int8ResidentArgs a)
 {
@@ -37,6 +87,13 @@ __global__ __launch_bounds__(RS_THREADS) void int8_resident_kernel(Int8ResidentA
         const u32 color = hh & 1u;
         const u32 t = a.t0 + (hh >> 1);
         const u32 row_lo = T - ext, n_rows = R + 2u * ext;
+        const u32 wpr = per_row / 64u;  // waves per row, when whole
+        if (per_row % 64u == 0u && (wpr & (wpr - 1u)) == 0u) {
+            if (wpr == 1u) int8_rs_rows<HB, true>(a, own, other, thr_lds, row_lo, n_rows, first, color, t);
+            else int8_rs_rows<HB, false>(a, own, other, thr_lds, row_lo, n_rows, first, color, t);
+            __syncthreads();
+            return;
+        }
         for (u32 it = threadIdx.x; it < n_rows * per_row; it += RS_THREADS) {
             const u32 rr = fast_div(it, a.div_row);
             const u32 row = row_lo + rr;  // LDS row, 1 <= row <= R + 2T - 2
