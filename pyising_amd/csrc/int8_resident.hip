@@ -11,7 +11,9 @@
 #include "resident.hpp"
 
 // Sites per thread and step: 8 (measured at L = 512 .. 2048 against 16 with 512 threads and 4
-// with 1024: more waves per CU hide the Philox and LDS latency of the short half sweeps).
+// with 1024: more waves per CU hide the Philox and LDS latency of the short half sweeps; 4 was
+// measured again with the tagged exchange: 3.27 against 3.18 us per sweep at 1024^2, 7.56
+// against 6.17 at 2048^2).
 constexpr u32 RS_V = 8;
 
 struct Int8ResidentArgs {

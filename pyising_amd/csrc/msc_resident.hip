@@ -7,15 +7,17 @@
 // leave the CUs between half sweeps. Measured (flips/ns, best per-half-sweep kernel against
 // resident, same box): 1024^2 113 / 176, 2048^2 429 / 672, 4096^2 1514 / 2036; at 8192^2 the
 // per-half-sweep compaction kernel is as fast (3722 / 3683). A band's half sweep holds few
-// words (L = 1024: at most 80 per workgroup), so each lockstep wave's chain of Philox calls,
-// not the VALU throughput, sets its time.
+// words (L = 1024: at most 80 per workgroup), so with one lane per word each lockstep wave's
+// chain of Philox calls, not the VALU throughput, sets its time; bands of at most
+// RS_LANES8_WORDS words per half sweep therefore give each word eight lanes (below).
 //
-// Each lane owns one word of a row: own, the other colour at the same word, the other colour's
-// words in the rows either side and the horizontal neighbour word from LDS, the same
-// classification (msc_state) and lockstep lazy comparison (msc_run) as the per-half-sweep
-// lockstep kernel, with the global word index in the Philox counter (word, sweep, colour,
-// 1 + level pair): bit-identical words (src/pyising/backends/numpy_backend.py:103-124 is the
-// reference's half sweep).
+// One lane per word: own, the other colour at the same word, the other colour's words in the
+// rows either side and the horizontal neighbour word from LDS, the same classification
+// (msc_state) and lockstep lazy comparison (msc_run) as the per-half-sweep lockstep kernel,
+// with the global word index in the Philox counter (word, sweep, colour, 1 + level pair):
+// bit-identical words (src/pyising/backends/numpy_backend.py:103-124 is the reference's half
+// sweep). Eight lanes per word: the same classification on every lane of the word's segment,
+// one level pair per lane, combined as described at rs_lanes8_combine.
 #include "msc_core.hpp"
 #include "resident.hpp"
 
