@@ -224,3 +224,24 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120, cwd=root)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_committed_pmc_profile_matches_the_kernel_sources():
+    """bench.py reports roofline.traffic from profiles/msc_L16384_pmc.json only when that
+    profile was measured on the msc kernel sources and flags built now (their digest is recorded
+    in it); a change to those sources must come with a new profile, or the line's traffic turns
+    null. Checked here so that the mismatch shows before a GPU run."""
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    sys.path.insert(0, str(root))
+    try:
+        import bench
+    finally:
+        sys.path.remove(str(root))
+    traffic, note = bench.traffic_from_profile()
+    assert traffic is not None, note
+    prof = json.loads((root / "profiles" / "msc_L16384_pmc.json").read_text())
+    assert traffic == prof["hbm_bytes_per_launch"]
+    assert 0 < traffic < prof["algorithmic_bytes_per_launch"]  # each XCD keeps its band in L2
